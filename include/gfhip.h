@@ -1,0 +1,165 @@
+/*
+ * gfhip -- MI355X (gfx950) implementation of the guarded-floor GMG-FGMRES
+ * solver path of solver-admissibility-and-guarded-floor-selection.
+ *
+ * C ABI: plain pointers and sizes.  Every pointer argument named d_* (or a
+ * vector argument of a compute call) is DEVICE memory; `stream` is a
+ * hipStream_t (NULL = default stream).  Every function returns 0 on success
+ * and a negative code on failure (-1 HIP error, -2 bad argument, -3 not set
+ * up); gf_last_error() returns a description of the last failure on the
+ * calling thread.  Calls on one object are not re-entrant.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   gf_ctx_*            MatrixFreeKff.__init__            src/gpu_fem/solver_v2.py:1153-1175
+ *   gf_matvec_f64/f32   MatrixFreeKff.matvec              src/gpu_fem/solver_v2.py:1182-1222
+ *   gf_matvec_f32, gf_matvec_full_f32  FusedMatvec.matvec / .matvec_full  cuda_fused_matvec.py:343-391
+ *   gf_diag_f64         MatrixFreeKff.extract_diagonal    src/gpu_fem/solver_v2.py:1228-1251
+ *   gf_nodeblock_*      NodeBlockJacobiMF.update / apply   src/gpu_fem/solver_v2.py:1381-1451
+ *   gf_gmg_create       GalerkinMatFreeGMG.__init__       src/gpu_fem/multigrid_v4.py:546-770
+ *   gf_gmg_setup        GalerkinMatFreeGMG.setup          src/gpu_fem/multigrid_v4.py:1119-1341
+ *   gf_gmg_apply        GalerkinMatFreeGMG.apply          src/gpu_fem/multigrid_v4.py:1759-1767
+ *   gf_gmg_vcycle       GalerkinMatFreeGMG._apply_base_cycle multigrid_v4.py:1737-1740
+ *   gf_gmg_level_*      GalerkinMatFreeGMG._K_gpu / _diag_inv_gpu (read-out for checks)
+ *   gf_gmg_solve        _cupy_fgmres(A=MatrixFreeKff.matvec(., E), M=gmg.apply)
+ *                                                         src/gpu_fem/multigrid_v4.py:416-534
+ *   gf_dot_f64 / gf_nrm2_f64  cp.dot / cp.linalg.norm used by _cupy_fgmres / _cupy_pcg
+ */
+#ifndef GFHIP_H
+#define GFHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gf_ctx gf_ctx;
+typedef struct gf_gmg gf_gmg;
+
+/* GalerkinMatFreeGMG keyword arguments (multigrid_v4.py:546-717).  Strings of
+ * the reference are encoded as small integers:
+ *   fine_smoother: 0 "fp64", 1 "fp32", 2 "bf16"
+ *   level_prec[i]: -1 default schedule, 0 "fp64", 1 "fp32", 2 "bf16"
+ *   smoother_type: 0 "chebyshev", 1 "jacobi"
+ *   cycle_type:    0 "v", 1 "w"
+ *   coarse_correction_policy: 0 "none", 1 "residual_line_search"
+ *   root_enrichment_mode: 0 "none", 1 "weighted_residual_second",
+ *       2 "weighted_gap_second", 3 "void_residual_second", 4 "void_gap_second"
+ *   root_local_correction_mode: 0 "none", 1 "node_block_line_search",
+ *       2 "node_block_inner_fgmres"                                        */
+typedef struct gf_gmg_params {
+  int32_t n_levels;
+  int32_t coarse_smooth_iters;
+  int32_t fine_smoother;
+  int32_t fine_smoother_degree;
+  double omega;
+  int32_t dense_chol_max;
+  int32_t coarse_pcg_iters;
+  int32_t smoother_type;
+  double cheb_lower_frac;
+  int32_t level_prec[8];
+  int32_t cycle_type;
+  int32_t coarse_correction_policy;
+  double coarse_correction_acceptance_ratio;
+  double coarse_correction_max_scale;
+  int32_t root_enrichment_mode;
+  double root_enrichment_weight_floor;
+  double root_enrichment_weight_power;
+  int32_t root_local_correction_mode;
+  double root_local_correction_max_scale;
+  int32_t root_local_correction_inner_steps;
+  int32_t root_local_correction_inner_restart;
+  int32_t inner_krylov_steps;
+  int32_t inner_krylov_restart;
+} gf_gmg_params;
+
+/* State read back after gf_gmg_setup (properties of GalerkinMatFreeGMG,
+ * multigrid_v4.py:771-836). */
+typedef struct gf_gmg_info {
+  int32_t n_levels;
+  int32_t n_free[8];
+  int32_t dims[8][3];
+  double lambda_max[8];
+  double omega_used;
+  double contrast_ratio;
+  double void_frac;
+  int32_t is_high_contrast;
+  int32_t fine_degree_adapt;
+  int32_t coarse_iters_adapt;
+  int32_t coarse_chol_mode; /* 0 none, 1 dense, 2 iterative */
+  int32_t coarse_chol_regularised;
+  int32_t level_prec[8];
+  int32_t setup_count;
+} gf_gmg_info;
+
+int32_t gf_version(void);
+const char* gf_last_error(void);
+
+/* ---- fine operator ---------------------------------------------------- */
+/* Structured hex grid nelx x nely x nelz with the reference node numbering
+ * (pub_simp_solver.py:196-207).  KE: 576 doubles (host), row-major 24x24.
+ * free: n_free sorted full-DOF indices (host).  ndof = 3*(nelx+1)(nely+1)(nelz+1). */
+int32_t gf_ctx_create(gf_ctx** out, int32_t nelx, int32_t nely, int32_t nelz,
+                      const double* KE, const int32_t* free_dofs, int32_t n_free, int32_t ndof);
+int32_t gf_ctx_destroy(gf_ctx* ctx);
+/* y_free = K_ff(E) u_free. E: n_elem moduli; u,y: n_free (device). */
+int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream);
+int32_t gf_matvec_f32(gf_ctx* ctx, const float* E, const float* u, float* y, void* stream);
+/* y = b - K_ff(E) u  (fused residual) */
+int32_t gf_residual_f64(gf_ctx* ctx, const double* E, const double* b, const double* u,
+                        double* y, void* stream);
+/* Full-length (ndof) variant: u_full must be zero on fixed DOFs; y_full gets
+ * the unrestricted K u on every DOF (FusedMatvec.matvec_full contract,
+ * cuda_fused_matvec.py:379-391). */
+int32_t gf_matvec_full_f32(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
+                           void* stream);
+int32_t gf_diag_f64(gf_ctx* ctx, const double* E, double* diag, void* stream);
+int32_t gf_nodeblock_update(gf_ctx* ctx, const double* E, const double* scalar_diag /*nullable*/,
+                            void* stream);
+int32_t gf_nodeblock_apply(gf_ctx* ctx, const double* r, double* z, void* stream);
+
+/* ---- BLAS-1 (deterministic fixed-order reductions) -------------------- */
+/* result written to a device double (d_out); ctx may be NULL (process-wide
+ * scratch, default-stream use only) */
+int32_t gf_dot_f64(gf_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out,
+                   void* stream);
+int32_t gf_nrm2_f64(gf_ctx* ctx, int64_t n, const double* x, double* d_out, void* stream);
+/* y = a*x + b*y (host scalars); the vector updates of _cupy_fgmres/_cupy_pcg
+ * when they run with caller-supplied operators */
+int32_t gf_axpby_f64(int64_t n, double a, const double* x, double b, double* y, void* stream);
+
+/* ---- Galerkin GMG hierarchy ------------------------------------------- */
+void gf_gmg_default_params(gf_gmg_params* p);
+int32_t gf_gmg_create(gf_gmg** out, gf_ctx* ctx, const gf_gmg_params* params);
+int32_t gf_gmg_destroy(gf_gmg* g);
+/* E: n_elem FP64 moduli (device); fine_diag: n_free or NULL. Synchronises. */
+int32_t gf_gmg_setup(gf_gmg* g, const double* E, const double* fine_diag, void* stream);
+int32_t gf_gmg_info_get(gf_gmg* g, gf_gmg_info* info);
+/* out = M^{-1} rhs (the full nonlinear preconditioner). No host sync. */
+int32_t gf_gmg_apply(gf_gmg* g, const double* rhs, double* out, void* stream);
+/* out = one base cycle (V or W) on rhs, no wrapper. */
+int32_t gf_gmg_vcycle(gf_gmg* g, const double* rhs, double* out, void* stream);
+/* y = K_l x at level l (l = 0: fine operator with the set-up moduli). */
+int32_t gf_gmg_level_matvec(gf_gmg* g, int32_t level, const double* x, double* y, void* stream);
+/* copy the level's inverse diagonal (n_free[level] doubles) */
+int32_t gf_gmg_level_dinv(gf_gmg* g, int32_t level, double* out, void* stream);
+/* dense copy (row-major n_free x n_free, device) of the level operator, lv >= 1 */
+int32_t gf_gmg_level_dense(gf_gmg* g, int32_t level, double* out, void* stream);
+/* restriction / prolongation between level and level+1 */
+int32_t gf_gmg_restrict(gf_gmg* g, int32_t level, const double* rf, double* rc, void* stream);
+int32_t gf_gmg_prolong(gf_gmg* g, int32_t level, const double* ec, double* ef, void* stream);
+/* free-DOF list of a level (host ints, n_free[level] entries) */
+int32_t gf_gmg_level_free(gf_gmg* g, int32_t level, int32_t* out_host);
+
+/* FGMRES(restart) with A = K_ff(E of the last setup), M = gf_gmg_apply.
+ * b, x: n_free device.  x0_given: use x as start vector.  history: host
+ * array receiving the relative residual history (capacity hist_cap). */
+int32_t gf_gmg_solve(gf_gmg* g, const double* b, double* x, int32_t x0_given, double tol,
+                     int32_t maxiter, int32_t restart, double* history, int32_t hist_cap,
+                     int32_t* n_hist, int32_t* iters, int32_t* converged, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GFHIP_H */

@@ -1,0 +1,379 @@
+// extern "C" entry points (declared in include/gfhip.h).
+#include "gf_solver.h"
+
+#include <new>
+
+using namespace gf;
+
+struct gf_ctx {
+  Ctx c;
+};
+struct gf_gmg {
+  Gmg g;
+};
+
+static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define CHECK_PTR(p)                                  \
+  do {                                                \
+    if (!(p)) {                                       \
+      set_error(std::string("null argument: ") + #p); \
+      return -2;                                      \
+    }                                                 \
+  } while (0)
+
+extern "C" {
+
+int32_t gf_version(void) { return 10000; }
+
+const char* gf_last_error(void) { return last_error(); }
+
+int32_t gf_ctx_create(gf_ctx** out, int32_t nelx, int32_t nely, int32_t nelz, const double* KE,
+                      const int32_t* free_dofs, int32_t n_free, int32_t ndof) {
+  CHECK_PTR(out);
+  CHECK_PTR(KE);
+  CHECK_PTR(free_dofs);
+  gf_ctx* c = new (std::nothrow) gf_ctx();
+  if (!c) {
+    set_error("gf_ctx_create: out of host memory");
+    return -1;
+  }
+  int rc = c->c.init(nelx, nely, nelz, KE, free_dofs, n_free, ndof);
+  if (rc) {
+    c->c.release();
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int32_t gf_ctx_destroy(gf_ctx* ctx) {
+  if (!ctx) return 0;
+  ctx->c.release();
+  delete ctx;
+  return 0;
+}
+
+int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  CHECK_PTR(u);
+  CHECK_PTR(y);
+  return fine_matvec<double>(ctx->c.g, ctx->c.map(), ctx->c.Ke, E, u, y, EPI_PLAIN,
+                             EpiArgs<double>{}, nullptr, S(stream));
+}
+
+int32_t gf_matvec_f32(gf_ctx* ctx, const float* E, const float* u, float* y, void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  CHECK_PTR(u);
+  CHECK_PTR(y);
+  return fine_matvec<float>(ctx->c.g, ctx->c.map(), ctx->c.Ke32, E, u, y, EPI_PLAIN,
+                            EpiArgs<float>{}, nullptr, S(stream));
+}
+
+int32_t gf_residual_f64(gf_ctx* ctx, const double* E, const double* b, const double* u,
+                        double* y, void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  CHECK_PTR(b);
+  CHECK_PTR(u);
+  CHECK_PTR(y);
+  EpiArgs<double> ep;
+  ep.b = b;
+  return fine_matvec<double>(ctx->c.g, ctx->c.map(), ctx->c.Ke, E, u, y, EPI_RESID, ep, nullptr,
+                             S(stream));
+}
+
+int32_t gf_matvec_full_f32(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
+                           void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  CHECK_PTR(u_full);
+  CHECK_PTR(y_full);
+  // full layout = identity DOF map: y = K u over all DOFs (fixed rows give
+  // the reaction K u there), the FusedMatvec.matvec_full contract
+  DofMap ident{0, nullptr};
+  return fine_matvec<float>(ctx->c.g, ident, ctx->c.Ke32, E, u_full, y_full, EPI_PLAIN,
+                            EpiArgs<float>{}, nullptr, S(stream));
+}
+
+int32_t gf_diag_f64(gf_ctx* ctx, const double* E, double* diag, void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  CHECK_PTR(diag);
+  return fine_diag(ctx->c.g, ctx->c.map(), ctx->c.Ke, E, diag, S(stream));
+}
+
+int32_t gf_nodeblock_update(gf_ctx* ctx, const double* E, const double* scalar_diag,
+                            void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  int rc = ctx->c.nodeblock_alloc();
+  if (rc) return rc;
+  ctx->c.nb_has_pinv = scalar_diag ? 1 : 0;
+  return nodeblock_update(ctx->c.g, ctx->c.map(), ctx->c.Ke, E, scalar_diag, ctx->c.nb_inv9,
+                          ctx->c.nb_pinv, S(stream));
+}
+
+int32_t gf_nodeblock_apply(gf_ctx* ctx, const double* r, double* z, void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(r);
+  CHECK_PTR(z);
+  if (!ctx->c.nb_inv9) {
+    set_error("gf_nodeblock_apply: update() has not been called");
+    return -3;
+  }
+  return nodeblock_apply(ctx->c.g, ctx->c.map(), ctx->c.nb_inv9, ctx->c.nb_pinv,
+                         ctx->c.nb_has_pinv, r, z, nullptr, S(stream));
+}
+
+// reductions without a context use one process-wide scratch (single stream)
+static Work* blas_work(gf_ctx* ctx) {
+  if (ctx) return &ctx->c.work;
+  static Work w;
+  if (w.init() != 0) return nullptr;
+  return &w;
+}
+
+int32_t gf_dot_f64(gf_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out,
+                   void* stream) {
+  CHECK_PTR(x);
+  CHECK_PTR(y);
+  CHECK_PTR(d_out);
+  Work* w = blas_work(ctx);
+  if (!w) return -1;
+  return dot_f64(*w, n, x, y, d_out, false, nullptr, S(stream));
+}
+
+int32_t gf_nrm2_f64(gf_ctx* ctx, int64_t n, const double* x, double* d_out, void* stream) {
+  CHECK_PTR(x);
+  CHECK_PTR(d_out);
+  Work* w = blas_work(ctx);
+  if (!w) return -1;
+  return dot_f64(*w, n, x, x, d_out, true, nullptr, S(stream));
+}
+
+__global__ void k_axpby_host(long n, double a, const double* __restrict__ x, double b,
+                             double* __restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = (b == 1.0 ? y[i] : b * y[i]) + a * x[i];
+}
+
+int32_t gf_axpby_f64(int64_t n, double a, const double* x, double b, double* y, void* stream) {
+  CHECK_PTR(x);
+  CHECK_PTR(y);
+  if (n <= 0) return 0;
+  int grid = cdiv(n, kBlock);
+  if (grid > 4096) grid = 4096;
+  hipLaunchKernelGGL(k_axpby_host, dim3(grid), dim3(kBlock), 0, S(stream), (long)n, a, x, b, y);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+void gf_gmg_default_params(gf_gmg_params* p) {
+  if (!p) return;
+  p->n_levels = 4;
+  p->coarse_smooth_iters = 2;
+  p->fine_smoother = 1;
+  p->fine_smoother_degree = 2;
+  p->omega = 0.5;
+  p->dense_chol_max = 5000;
+  p->coarse_pcg_iters = 80;
+  p->smoother_type = 0;
+  p->cheb_lower_frac = 1.0 / 30.0;
+  for (int i = 0; i < 8; ++i) p->level_prec[i] = -1;
+  p->cycle_type = 0;
+  p->coarse_correction_policy = 0;
+  p->coarse_correction_acceptance_ratio = 1.0;
+  p->coarse_correction_max_scale = 1.0;
+  p->root_enrichment_mode = 0;
+  p->root_enrichment_weight_floor = 1e-6;
+  p->root_enrichment_weight_power = 0.5;
+  p->root_local_correction_mode = 0;
+  p->root_local_correction_max_scale = 1.0;
+  p->root_local_correction_inner_steps = 10;
+  p->root_local_correction_inner_restart = 0;
+  p->inner_krylov_steps = 0;
+  p->inner_krylov_restart = 0;
+}
+
+int32_t gf_gmg_create(gf_gmg** out, gf_ctx* ctx, const gf_gmg_params* params) {
+  CHECK_PTR(out);
+  CHECK_PTR(ctx);
+  CHECK_PTR(params);
+  if (params->inner_krylov_restart > 1024 || params->root_local_correction_inner_restart > 1024) {
+    set_error("gf_gmg_create: inner restarts are limited to 1024");
+    return -2;
+  }
+  gf_gmg* g = new (std::nothrow) gf_gmg();
+  if (!g) {
+    set_error("gf_gmg_create: out of host memory");
+    return -1;
+  }
+  int rc = g->g.create(&ctx->c, params);
+  if (rc) {
+    g->g.release();
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return 0;
+}
+
+int32_t gf_gmg_destroy(gf_gmg* g) {
+  if (!g) return 0;
+  g->g.release();
+  delete g;
+  return 0;
+}
+
+int32_t gf_gmg_setup(gf_gmg* g, const double* E, const double* fine_diag, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(E);
+  return g->g.setup(E, fine_diag, S(stream));
+}
+
+int32_t gf_gmg_info_get(gf_gmg* g, gf_gmg_info* info) {
+  CHECK_PTR(g);
+  CHECK_PTR(info);
+  Gmg& G = g->g;
+  *info = gf_gmg_info{};
+  info->n_levels = G.L;
+  for (int l = 0; l < G.L; ++l) {
+    info->n_free[l] = G.lv[l].n_free;
+    info->dims[l][0] = G.lv[l].g.nx;
+    info->dims[l][1] = G.lv[l].g.ny;
+    info->dims[l][2] = G.lv[l].g.nz;
+    info->lambda_max[l] = l < (int)G.lam.size() ? G.lam[l] : 0.0;
+    info->level_prec[l] = G.lv[l].prec;
+  }
+  info->omega_used = G.omega_used;
+  info->contrast_ratio = G.contrast;
+  info->void_frac = G.void_frac;
+  info->is_high_contrast = G.high;
+  info->fine_degree_adapt = G.fine_deg;
+  info->coarse_iters_adapt = G.coarse_it;
+  info->coarse_chol_mode = G.chol_mode;
+  info->coarse_chol_regularised = G.chol_regularised;
+  info->setup_count = G.setup_count;
+  return 0;
+}
+
+int32_t gf_gmg_apply(gf_gmg* g, const double* rhs, double* out, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(rhs);
+  CHECK_PTR(out);
+  return g->g.apply(rhs, out, S(stream));
+}
+
+int32_t gf_gmg_vcycle(gf_gmg* g, const double* rhs, double* out, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(rhs);
+  CHECK_PTR(out);
+  if (!g->g.is_setup) {
+    set_error("GalerkinMatFreeGMG.setup() must be called before apply().");
+    return -3;
+  }
+  return g->g.base_cycle(rhs, out, nullptr, S(stream));
+}
+
+static int check_level(gf_gmg* g, int level, bool need_coarse) {
+  if (!g->g.is_setup) {
+    set_error("GalerkinMatFreeGMG.setup() must be called first");
+    return -3;
+  }
+  if (level < (need_coarse ? 1 : 0) || level >= g->g.L) {
+    set_error("level out of range");
+    return -2;
+  }
+  return 0;
+}
+
+int32_t gf_gmg_level_matvec(gf_gmg* g, int32_t level, const double* x, double* y, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(x);
+  CHECK_PTR(y);
+  int rc = check_level(g, level, false);
+  if (rc) return rc;
+  return g->g.level_op(level, x, y, EPI_PLAIN, EpiArgs<double>{}, nullptr, S(stream));
+}
+
+int32_t gf_gmg_level_dinv(gf_gmg* g, int32_t level, double* out, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(out);
+  int rc = check_level(g, level, false);
+  if (rc) return rc;
+  GF_HIP(hipMemcpyAsync(out, g->g.lv[level].dinv, sizeof(double) * g->g.lv[level].n_free,
+                        hipMemcpyDeviceToDevice, S(stream)));
+  return 0;
+}
+
+int32_t gf_gmg_level_dense(gf_gmg* g, int32_t level, double* out, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(out);
+  int rc = check_level(g, level, true);
+  if (rc) return rc;
+  Level& v = g->g.lv[level];
+  return stencil_to_dense(v.g, v.map(), v.A, v.n_free, out, S(stream));
+}
+
+int32_t gf_gmg_restrict(gf_gmg* g, int32_t level, const double* rf, double* rc_, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(rf);
+  CHECK_PTR(rc_);
+  if (level < 0 || level + 1 >= g->g.L) {
+    set_error("level out of range");
+    return -2;
+  }
+  Level& f = g->g.lv[level];
+  Level& c = g->g.lv[level + 1];
+  return restrict_(f.g, f.map(), c.g, c.map(), rf, rc_, nullptr, S(stream));
+}
+
+int32_t gf_gmg_prolong(gf_gmg* g, int32_t level, const double* ec, double* ef, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(ec);
+  CHECK_PTR(ef);
+  if (level < 0 || level + 1 >= g->g.L) {
+    set_error("level out of range");
+    return -2;
+  }
+  Level& f = g->g.lv[level];
+  Level& c = g->g.lv[level + 1];
+  return prolong(f.g, f.map(), c.g, c.map(), ec, ef, nullptr, S(stream));
+}
+
+int32_t gf_gmg_level_free(gf_gmg* g, int32_t level, int32_t* out_host) {
+  CHECK_PTR(g);
+  CHECK_PTR(out_host);
+  if (level < 0 || level >= g->g.L) {
+    set_error("level out of range");
+    return -2;
+  }
+  const std::vector<int>& f = g->g.lv[level].h_free;
+  for (size_t i = 0; i < f.size(); ++i) out_host[i] = f[i];
+  return 0;
+}
+
+int32_t gf_gmg_solve(gf_gmg* g, const double* b, double* x, int32_t x0_given, double tol,
+                     int32_t maxiter, int32_t restart, double* history, int32_t hist_cap,
+                     int32_t* n_hist, int32_t* iters, int32_t* converged, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(b);
+  CHECK_PTR(x);
+  CHECK_PTR(n_hist);
+  CHECK_PTR(iters);
+  CHECK_PTR(converged);
+  int nh = 0, it = 0, cv = 0;
+  int rc = g->g.outer_solve(b, x, x0_given, tol, maxiter, restart, history, hist_cap, &nh, &it,
+                            &cv, S(stream));
+  *n_hist = nh;
+  *iters = it;
+  *converged = cv;
+  return rc;
+}
+
+}  // extern "C"

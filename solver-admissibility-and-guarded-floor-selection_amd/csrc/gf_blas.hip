@@ -1,0 +1,378 @@
+// Deterministic BLAS-1 building blocks with device-resident scalars.
+//
+// Every reduction uses a fixed grid of at most kRedBlocks blocks, a fixed
+// in-block tree and a ticket so that the last block to finish folds the block
+// partials in index order: the result is bitwise reproducible for a given n
+// and never needs a host round trip.  All kernels accept an optional `gate`
+// (device int); when it points at 0 the kernel is a no-op.  The inner Krylov
+// loops use this to stop after a breakdown without host synchronisation.
+#include "gf_internal.h"
+
+namespace gf {
+
+std::string& err_slot() {
+  static thread_local std::string s;
+  return s;
+}
+void set_error(const std::string& msg) { err_slot() = msg; }
+const char* last_error() { return err_slot().c_str(); }
+
+// ---------------------------------------------------------------------------
+// block reduction helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Sum over a 256-thread block; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return r;
+}
+
+// Fold NP partial sums of every block into out[0..NP) (last block only).
+// Returns true in the block that performed the fold.
+template <int NP>
+__device__ __forceinline__ void finish_reduce(const double (&v)[NP], double* partials,
+                                              int* ticket, double* out, int post) {
+  __shared__ double lds[4];
+  __shared__ bool last;
+  double bs[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) bs[p] = block_sum(v[p], lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) partials[p * kRedBlocks + blockIdx.x] = bs[p];
+    __threadfence();
+    int t = atomicAdd(ticket, 1);
+    last = (t == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int nb = gridDim.x;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+      a += __builtin_nontemporal_load(&partials[p * kRedBlocks + i]);
+    double s = block_sum(a, lds);
+    if (threadIdx.x == 0) {
+      if (post == 1) s = sqrt(s);
+      out[p] = s;
+    }
+  }
+  if (threadIdx.x == 0) *ticket = 0;
+}
+
+inline int red_grid(long n) {
+  int g = cdiv(n, kBlock * 4);
+  return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(long n, const T* __restrict__ x,
+                                                const T* __restrict__ y, double* partials,
+                                                int* ticket, double* out, int post,
+                                                const int* gate) {
+  if (gate && *gate == 0) return;
+  double a[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    a[0] += (double)x[i] * (double)y[i];
+  finish_reduce<1>(a, partials, ticket, out, post);
+}
+
+// out[0]=x.x  out[1]=y.y  out[2]=z.z  out[3]=y.z  (line-search statistics)
+__global__ __launch_bounds__(kBlock) void k_dot4_ls(long n, const double* __restrict__ c,
+                                                    const double* __restrict__ r,
+                                                    const double* __restrict__ Ac,
+                                                    double* partials, int* ticket, double* out,
+                                                    const int* gate) {
+  if (gate && *gate == 0) return;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    double cv = c[i], rv = r[i], av = Ac[i];
+    a[0] += cv * cv;
+    a[1] += rv * rv;
+    a[2] += av * av;
+    a[3] += rv * av;
+  }
+  finish_reduce<4>(a, partials, ticket, out, 0);
+}
+
+// w -= h*v (h = *hptr) and out = next . w  (fused MGS step)
+__global__ __launch_bounds__(kBlock) void k_mgs_axpy_dot(long n, const double* __restrict__ v,
+                                                         double* __restrict__ w,
+                                                         const double* hptr,
+                                                         const double* __restrict__ next,
+                                                         double* partials, int* ticket,
+                                                         double* out, int post, const int* gate) {
+  if (gate && *gate == 0) return;
+  const double h = *hptr;
+  double a[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    double wv = w[i] - h * v[i];
+    w[i] = wv;
+    a[0] += (next ? next[i] : wv) * wv;
+  }
+  finish_reduce<1>(a, partials, ticket, out, post);
+}
+
+// y = b - s*x ; out = ||y||  with s = *sptr (trial residual norm, y may be null)
+__global__ __launch_bounds__(kBlock) void k_trial_norm(long n, const double* __restrict__ b,
+                                                       const double* __restrict__ x,
+                                                       const double* sptr, double* y,
+                                                       double* partials, int* ticket,
+                                                       double* out, const int* gate) {
+  if (gate && *gate == 0) return;
+  const double s = *sptr;
+  double a[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    double t = b[i] - s * x[i];
+    if (y) y[i] = t;
+    a[0] += t * t;
+  }
+  finish_reduce<1>(a, partials, ticket, out, 1);
+}
+
+// y = a*x + b*y  with scalars read from device (aptr/bptr may be null = 0)
+__global__ __launch_bounds__(kBlock) void k_axpby_dev(long n, const double* aptr, double asign,
+                                                      const double* __restrict__ x,
+                                                      const double* bptr, double* __restrict__ y,
+                                                      const int* gate) {
+  if (gate && *gate == 0) return;
+  const double a = asign * *aptr;
+  const double bb = bptr ? *bptr : 1.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = (bptr ? bb * y[i] : y[i]) + a * x[i];
+}
+
+// y = x / h  if h > thresh  (Arnoldi normalisation; otherwise y untouched)
+__global__ __launch_bounds__(kBlock) void k_scale_inv(long n, const double* __restrict__ x,
+                                                      const double* hptr, double thresh,
+                                                      double* __restrict__ y, const int* gate) {
+  if (gate && *gate == 0) return;
+  const double h = *hptr;
+  if (!(h > thresh)) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = x[i] / h;
+}
+
+// x = sum_j y[j] * Z[j]   (j < *jcount), accumulated in j order from 0
+__global__ __launch_bounds__(kBlock) void k_combine(long n, const double* __restrict__ Z,
+                                                    long ldz, const double* ycoef,
+                                                    const int* jcount, int accumulate,
+                                                    double* __restrict__ x, const int* gate) {
+  if (gate && *gate == 0) return;
+  const int m = *jcount;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    double acc = accumulate ? x[i] : 0.0;
+    for (int j = 0; j < m; ++j) acc = acc + ycoef[j] * Z[j * ldz + i];
+    x[i] = acc;
+  }
+}
+
+template <typename TO, typename TI>
+__global__ __launch_bounds__(kBlock) void k_cast(long n, const TI* __restrict__ x,
+                                                 TO* __restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = (TO)x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sub(long n, const double* __restrict__ a,
+                                                const double* __restrict__ b,
+                                                double* __restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    y[i] = a[i] - b[i];
+}
+
+// E statistics: out[0]=max(E) out[1]=min(E) (exact, order-free)
+__global__ __launch_bounds__(kBlock) void k_minmax(long n, const double* __restrict__ E,
+                                                   double* out2, double* partials, int* ticket) {
+  __shared__ double smax[kBlock], smin[kBlock];
+  __shared__ bool last;
+  double mx = -INFINITY, mn = INFINITY;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    mx = fmax(mx, E[i]);
+    mn = fmin(mn, E[i]);
+  }
+  smax[threadIdx.x] = mx;
+  smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = smax[0];
+    partials[kRedBlocks + blockIdx.x] = smin[0];
+    __threadfence();
+    last = (atomicAdd(ticket, 1) == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double a = -INFINITY, b = INFINITY;
+    for (int i = 0; i < (int)gridDim.x; ++i) {
+      a = fmax(a, __builtin_nontemporal_load(&partials[i]));
+      b = fmin(b, __builtin_nontemporal_load(&partials[kRedBlocks + i]));
+    }
+    out2[0] = a;
+    out2[1] = b;
+    *ticket = 0;
+  }
+}
+
+// count of E > thr   (thr read from device: *thr_src * scale)
+__global__ __launch_bounds__(kBlock) void k_count_gt(long n, const double* __restrict__ E,
+                                                     const double* thr_src, double scale,
+                                                     double* partials, int* ticket, double* out) {
+  const double thr = (double)(*thr_src * scale);
+  double a[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    a[0] += (E[i] > thr) ? 1.0 : 0.0;
+  finish_reduce<1>(a, partials, ticket, out, 0);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+int dot_f64(Work& w, long n, const double* x, const double* y, double* out, bool sqrt_,
+            const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot<double>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, y,
+                     w.partials, w.ticket, out, sqrt_ ? 1 : 0, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int dot_f32(Work& w, long n, const float* x, const float* y, double* out, bool sqrt_,
+            const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot<float>, dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, y, w.partials,
+                     w.ticket, out, sqrt_ ? 1 : 0, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int dot4_ls(Work& w, long n, const double* c, const double* r, const double* Ac, double* out4,
+            const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot4_ls, dim3(red_grid(n)), dim3(kBlock), 0, st, n, c, r, Ac, w.partials,
+                     w.ticket, out4, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int mgs_axpy_dot(Work& w, long n, const double* v, double* wv, const double* h,
+                 const double* next, double* out, bool sqrt_, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_mgs_axpy_dot, dim3(red_grid(n)), dim3(kBlock), 0, st, n, v, wv, h, next,
+                     w.partials, w.ticket, out, sqrt_ ? 1 : 0, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int trial_norm(Work& w, long n, const double* b, const double* x, const double* s, double* y,
+               double* out, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_trial_norm, dim3(red_grid(n)), dim3(kBlock), 0, st, n, b, x, s, y,
+                     w.partials, w.ticket, out, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int axpby_dev(long n, const double* a, double asign, const double* x, const double* b, double* y,
+              const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpby_dev, dim3(red_grid(n)), dim3(kBlock), 0, st, n, a, asign, x, b, y,
+                     gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int scale_inv(long n, const double* x, const double* h, double thresh, double* y,
+              const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_inv, dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, h, thresh, y,
+                     gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int combine(long n, const double* Z, long ldz, const double* y, const int* jcount, bool acc,
+            double* x, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_combine, dim3(red_grid(n)), dim3(kBlock), 0, st, n, Z, ldz, y, jcount,
+                     acc ? 1 : 0, x, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int cast_f64_to_f32(long n, const double* x, float* y, hipStream_t st) {
+  hipLaunchKernelGGL((k_cast<float, double>), dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, y);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int cast_f32_to_f64(long n, const float* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, y);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int sub_f64(long n, const double* a, const double* b, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_sub, dim3(red_grid(n)), dim3(kBlock), 0, st, n, a, b, y);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int minmax_f64(Work& w, long n, const double* E, double* out2, hipStream_t st) {
+  hipLaunchKernelGGL(k_minmax, dim3(red_grid(n)), dim3(kBlock), 0, st, n, E, out2, w.partials,
+                     w.ticket);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int count_gt(Work& w, long n, const double* E, const double* thr_src, double scale, double* out,
+             hipStream_t st) {
+  hipLaunchKernelGGL(k_count_gt, dim3(red_grid(n)), dim3(kBlock), 0, st, n, E, thr_src, scale,
+                     w.partials, w.ticket, out);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int Work::init() {
+  if (partials) return 0;
+  GF_HIP(hipMalloc(&partials, sizeof(double) * 4 * kRedBlocks));
+  GF_HIP(hipMalloc(&ticket, sizeof(int) * 4));
+  GF_HIP(hipMemset(ticket, 0, sizeof(int) * 4));
+  return 0;
+}
+
+void Work::release() {
+  if (partials) (void)hipFree(partials);
+  if (ticket) (void)hipFree(ticket);
+  partials = nullptr;
+  ticket = nullptr;
+}
+
+}  // namespace gf

@@ -1,0 +1,702 @@
+// Coarse levels: element-wise level-1 Galerkin assembly, Galerkin R(KP)
+// products, block-stencil operators, trilinear transfers, the dense coarsest
+// factorisation and the fixed-count coarsest PCG.
+//
+// Every coarse operator is a 27-point stencil of 3x3 blocks on its node grid
+// (SoA layout, see gf_common.h).  Entries whose row or column DOF is fixed are
+// stored as exact zeros, which reproduces the reference's free-restricted
+// CSR matrices value for value.
+#include "gf_internal.h"
+
+namespace gf {
+
+__host__ __device__ constexpr int lidx_c(int px, int py, int pz) {
+  return pz * 4 + (py ? (px ? 2 : 3) : (px ? 1 : 0));
+}
+
+__device__ __forceinline__ void node_xyz(const Grid& g, int n, int& ix, int& iy, int& iz) {
+  iz = n % g.NZ;
+  const int t = n / g.NZ;
+  iy = t % g.NY;
+  ix = t / g.NY;
+}
+
+// ---------------------------------------------------------------------------
+// level-1 assembly: one thread per (coarse node I, stencil slot k)
+//   K1[I, I+d_k] = sum_{coarse elements C containing both, ascending}
+//                  sum_{stu in s-major order} E[fine(C,stu)] Q_stu[(I in C), (J in C)]
+// fine(C,stu) = min(2C+stu, nel_f - 1) per axis (odd fine dims drop the last
+// fine layer, exactly as the reference's element-wise construction).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_level1(Grid gc, DofMap mapc, Grid gf,
+                                                   const double* __restrict__ Q,
+                                                   const double* __restrict__ E,
+                                                   double* __restrict__ A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (n >= gc.n_nodes) return;
+  const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+  int ix, iy, iz;
+  node_xyz(gc, n, ix, iy, iz);
+  const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+  const bool jin = (jx >= 0) & (jx < gc.NX) & (jy >= 0) & (jy < gc.NY) & (jz >= 0) &
+                   (jz < gc.NZ);
+  const int m = (jx * gc.NY + jy) * gc.NZ + jz;
+  int orow[3], ocol[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    orow[q] = mapc(3 * n + q);
+    ocol[q] = jin ? mapc(3 * m + q) : -1;
+  }
+  double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (jin) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          // element C = (ix-1+a, iy-1+b, iz-1+c); J's local position 1+d-a must be 0/1
+          const int pjx = 1 + dx - a, pjy = 1 + dy - b, pjz = 1 + dz - c;
+          if (pjx < 0 || pjx > 1 || pjy < 0 || pjy > 1 || pjz < 0 || pjz > 1) continue;
+          const int cx = ix - 1 + a, cy = iy - 1 + b, cz = iz - 1 + c;
+          if (cx < 0 || cx >= gc.nx || cy < 0 || cy >= gc.ny || cz < 0 || cz >= gc.nz) continue;
+          const int li = lidx_c(1 - a, 1 - b, 1 - c);
+          const int lj = lidx_c(pjx, pjy, pjz);
+          double Ef[8];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            const int su = s >> 2, tu = (s >> 1) & 1, uu = s & 1;
+            const int fx = min(2 * cx + su, gf.nx - 1);
+            const int fy = min(2 * cy + tu, gf.ny - 1);
+            const int fz = min(2 * cz + uu, gf.nz - 1);
+            Ef[s] = E[(fx * gf.ny + fy) * gf.nz + fz];
+          }
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              double raw = 0.0;
+#pragma unroll
+              for (int s = 0; s < 8; ++s)
+                raw += Ef[s] * Q[s * 576 + (3 * li + r) * 24 + 3 * lj + q];
+              blk[r * 3 + q] += raw;
+            }
+        }
+  }
+  const long nn = gc.n_nodes;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const bool keep = (orow[r] >= 0) & (ocol[q] >= 0);
+      A[(long)(k * 9 + r * 3 + q) * nn + n] = keep ? blk[r * 3 + q] : 0.0;
+    }
+}
+
+int level1_assemble(const Grid& gc, const DofMap& mapc, const Grid& gf, const double* Q,
+                    const double* E, double* A, hipStream_t st) {
+  hipLaunchKernelGGL(k_level1, dim3(cdiv(gc.n_nodes, kBlock), 27), dim3(kBlock), 0, st, gc, mapc,
+                     gf, Q, E, A);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// block-stencil operator with the shared epilogues
+// ---------------------------------------------------------------------------
+template <typename T, int EPI>
+__device__ __forceinline__ void coarse_epi(int i, T ax, const T* __restrict__ x,
+                                           T* __restrict__ y, const EpiArgs<T>& ep) {
+  if (EPI == EPI_PLAIN) {
+    y[i] = ax;
+  } else if (EPI == EPI_RESID) {
+    y[i] = ep.b[i] - ax;
+  } else if (EPI == EPI_CHEB0) {
+    T z = ep.dinv[i] * (ep.b[i] - ax);
+    ep.dv[i] = z;
+    ep.xo[i] = x[i] + ep.a * z;
+  } else if (EPI == EPI_CHEB) {
+    T z = ep.dinv[i] * (ep.b[i] - ax);
+    T d = ep.a * (z + ep.c * ep.dv[i]);
+    ep.dv[i] = d;
+    ep.xo[i] = x[i] + d;
+  } else {
+    ep.xo[i] = x[i] + (ep.a * ep.dinv[i]) * (ep.b[i] - ax);
+  }
+}
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T* __restrict__ A,
+                                                    const T* __restrict__ x, T* __restrict__ y,
+                                                    EpiArgs<T> ep, const int* gate) {
+  if (gate && *gate == 0) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
+  if ((o0 & o1 & o2) < 0) return;
+  int ix, iy, iz;
+  node_xyz(g, n, ix, iy, iz);
+  const long nn = g.n_nodes;
+  T acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+    if (jx < 0 || jx >= g.NX || jy < 0 || jy >= g.NY || jz < 0 || jz >= g.NZ) continue;
+    const int m = (jx * g.NY + jy) * g.NZ + jz;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int f = map(3 * m + q);
+      if (f < 0) continue;
+      const T xv = x[f];
+      acc0 += A[(k * 9 + 0 * 3 + q) * nn + n] * xv;
+      acc1 += A[(k * 9 + 1 * 3 + q) * nn + n] * xv;
+      acc2 += A[(k * 9 + 2 * 3 + q) * nn + n] * xv;
+    }
+  }
+  if (o0 >= 0) coarse_epi<T, EPI>(o0, acc0, x, y, ep);
+  if (o1 >= 0) coarse_epi<T, EPI>(o1, acc1, x, y, ep);
+  if (o2 >= 0) coarse_epi<T, EPI>(o2, acc2, x, y, ep);
+}
+
+template <typename T>
+int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
+                   const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
+  dim3 grid(cdiv(g.n_nodes, kBlock)), block(kBlock);
+  switch (epi) {
+    case EPI_PLAIN:
+      hipLaunchKernelGGL((k_stencil<T, EPI_PLAIN>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+      break;
+    case EPI_RESID:
+      hipLaunchKernelGGL((k_stencil<T, EPI_RESID>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+      break;
+    case EPI_CHEB0:
+      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB0>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+      break;
+    case EPI_CHEB:
+      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+      break;
+    case EPI_JACOBI:
+      hipLaunchKernelGGL((k_stencil<T, EPI_JACOBI>), grid, block, 0, st, g, map, A, x, y, ep,
+                         gate);
+      break;
+    default:
+      set_error("stencil_matvec: bad epilogue");
+      return -2;
+  }
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+template int stencil_matvec<double>(const Grid&, const DofMap&, const double*, const double*,
+                                    double*, int, const EpiArgs<double>&, const int*,
+                                    hipStream_t);
+template int stencil_matvec<float>(const Grid&, const DofMap&, const float*, const float*,
+                                   float*, int, const EpiArgs<float>&, const int*, hipStream_t);
+
+__global__ __launch_bounds__(kBlock) void k_stencil_diag(Grid g, DofMap map,
+                                                         const double* __restrict__ A,
+                                                         double* __restrict__ d) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const long nn = g.n_nodes;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int f = map(3 * n + q);
+    if (f >= 0) d[f] = A[(13 * 9 + q * 4) * nn + n];
+  }
+}
+
+int stencil_diag(const Grid& g, const DofMap& map, const double* A, double* diag,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_stencil_diag, dim3(cdiv(g.n_nodes, kBlock)), dim3(kBlock), 0, st, g, map,
+                     A, diag);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(long n, const double* __restrict__ a,
+                                                   float* __restrict__ b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+int f64_to_f32_block(long n, const double* A, float* A32, hipStream_t st) {
+  int grid = cdiv(n, kBlock);
+  if (grid > 8192) grid = 8192;
+  hipLaunchKernelGGL(k_to_f32, dim3(grid), dim3(kBlock), 0, st, n, A, A32);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// trilinear transfers (row-normalised scalar weights, applied per component)
+// ---------------------------------------------------------------------------
+// parents of fine index i along one axis: up to 2 (index, weight) pairs, in
+// ascending index order, with the 1-D row sum (a power of two).
+__device__ __forceinline__ int parents_1d(int i, int ncn, int* p, double* w, double& s) {
+  int cnt = 0;
+  s = 0.0;
+  if ((i & 1) == 0) {
+    if (i / 2 < ncn) { p[cnt] = i / 2; w[cnt] = 1.0; ++cnt; }
+  } else {
+    if (i / 2 < ncn) { p[cnt] = i / 2; w[cnt] = 0.5; ++cnt; }
+    if (i / 2 + 1 < ncn) { p[cnt] = i / 2 + 1; w[cnt] = 0.5; ++cnt; }
+  }
+  for (int c = 0; c < cnt; ++c) s += w[c];
+  return cnt;
+}
+
+__device__ __forceinline__ double pweight(int i, int I, int ncn) {
+  int p[2];
+  double w[2], s;
+  const int cnt = parents_1d(i, ncn, p, w, s);
+  for (int c = 0; c < cnt; ++c)
+    if (p[c] == I) return w[c] / s;
+  return 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
+                                                    const double* __restrict__ ec,
+                                                    double* __restrict__ yf, const int* gate) {
+  if (gate && *gate == 0) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= gf.n_nodes) return;
+  const int o0 = mapf(3 * n), o1 = mapf(3 * n + 1), o2 = mapf(3 * n + 2);
+  if ((o0 & o1 & o2) < 0) return;
+  int ix, iy, iz;
+  node_xyz(gf, n, ix, iy, iz);
+  int px[2], py[2], pz[2];
+  double wx[2], wy[2], wz[2], sx, sy, sz;
+  const int cx = parents_1d(ix, gc.NX, px, wx, sx);
+  const int cy = parents_1d(iy, gc.NY, py, wy, sy);
+  const int cz = parents_1d(iz, gc.NZ, pz, wz, sz);
+  const double inv = 1.0 / ((sx * sy) * sz);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < cx; ++a)
+    for (int b = 0; b < cy; ++b)
+      for (int c = 0; c < cz; ++c) {
+        const double w = ((wx[a] * wy[b]) * wz[c]) * inv;
+        const int m = (px[a] * gc.NY + py[b]) * gc.NZ + pz[c];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int f = mapc(3 * m + q);
+          if (f >= 0) acc[q] += w * ec[f];
+        }
+      }
+  if (o0 >= 0) yf[o0] = acc[0];
+  if (o1 >= 0) yf[o1] = acc[1];
+  if (o2 >= 0) yf[o2] = acc[2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
+                                                     const double* __restrict__ rf,
+                                                     double* __restrict__ yc, const int* gate) {
+  if (gate && *gate == 0) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= gc.n_nodes) return;
+  const int o0 = mapc(3 * n), o1 = mapc(3 * n + 1), o2 = mapc(3 * n + 2);
+  if ((o0 & o1 & o2) < 0) return;
+  int ix, iy, iz;
+  node_xyz(gc, n, ix, iy, iz);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int a = -1; a <= 1; ++a) {
+    const int fx = 2 * ix + a;
+    if (fx < 0 || fx >= gf.NX) continue;
+    const double wxv = pweight(fx, ix, gc.NX);
+    if (wxv == 0.0) continue;
+    for (int b = -1; b <= 1; ++b) {
+      const int fy = 2 * iy + b;
+      if (fy < 0 || fy >= gf.NY) continue;
+      const double wyv = pweight(fy, iy, gc.NY);
+      if (wyv == 0.0) continue;
+      for (int c = -1; c <= 1; ++c) {
+        const int fz = 2 * iz + c;
+        if (fz < 0 || fz >= gf.NZ) continue;
+        const double wzv = pweight(fz, iz, gc.NZ);
+        if (wzv == 0.0) continue;
+        const double w = (wxv * wyv) * wzv;
+        const int m = (fx * gf.NY + fy) * gf.NZ + fz;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int f = mapf(3 * m + q);
+          if (f >= 0) acc[q] += w * rf[f];
+        }
+      }
+    }
+  }
+  if (o0 >= 0) yc[o0] = acc[0];
+  if (o1 >= 0) yc[o1] = acc[1];
+  if (o2 >= 0) yc[o2] = acc[2];
+}
+
+int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
+            const double* ec, double* yf, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_prolong, dim3(cdiv(gf.n_nodes, kBlock)), dim3(kBlock), 0, st, gf, mapf,
+                     gc, mapc, ec, yf, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int restrict_(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
+              const double* rf, double* yc, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_restrict, dim3(cdiv(gc.n_nodes, kBlock)), dim3(kBlock), 0, st, gf, mapf,
+                     gc, mapc, rf, yc, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Galerkin product K_c = R (K_f P), in two stages with the association of the
+// reference's sparse products (K_f P first, then R times that; every sum runs
+// in ascending column/row order).
+//   stage 1: KP[a, slot q] (3x3) = sum_b K_f[a,b] P[b, J_q],
+//            J_q = ceil((a-2)/2) + q per axis, q in {0,1,2}^3
+//   stage 2: K_c[I, I+d_k] = sum_{a child of I, ascending} P[a,I] KP[a, J]
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int jbase(int a) { return (a - 1) >> 1; }  // ceil((a-2)/2)
+
+__global__ __launch_bounds__(kBlock) void k_rap_kp(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
+                                                   const double* __restrict__ Af,
+                                                   double* __restrict__ KP) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (n >= gf.n_nodes) return;
+  int ax, ay, az;
+  node_xyz(gf, n, ax, ay, az);
+  const int Jx = jbase(ax) + q / 9, Jy = jbase(ay) + (q / 3) % 3, Jz = jbase(az) + q % 3;
+  const long nnf = gf.n_nodes;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const bool Jin = (Jx >= 0) & (Jx < gc.NX) & (Jy >= 0) & (Jy < gc.NY) & (Jz >= 0) &
+                   (Jz < gc.NZ);
+  int cfree[3] = {-1, -1, -1};
+  if (Jin) {
+    const int mJ = (Jx * gc.NY + Jy) * gc.NZ + Jz;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cfree[c] = mapc(3 * mJ + c);
+    for (int k = 0; k < 27; ++k) {
+      const int bx = ax + k / 9 - 1, by = ay + (k / 3) % 3 - 1, bz = az + k % 3 - 1;
+      if (bx < 0 || bx >= gf.NX || by < 0 || by >= gf.NY || bz < 0 || bz >= gf.NZ) continue;
+      const double w = (pweight(bx, Jx, gc.NX) * pweight(by, Jy, gc.NY)) * pweight(bz, Jz, gc.NZ);
+      if (w == 0.0) continue;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          acc[r * 3 + c] += Af[(long)(k * 9 + r * 3 + c) * nnf + n] * w;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      KP[(long)(q * 9 + r * 3 + c) * nnf + n] = (cfree[c] >= 0) ? acc[r * 3 + c] : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rap_r(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
+                                                  const double* __restrict__ KP,
+                                                  double* __restrict__ Ac) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (n >= gc.n_nodes) return;
+  int ix, iy, iz;
+  node_xyz(gc, n, ix, iy, iz);
+  const int Jx = ix + k / 9 - 1, Jy = iy + (k / 3) % 3 - 1, Jz = iz + k % 3 - 1;
+  const bool Jin = (Jx >= 0) & (Jx < gc.NX) & (Jy >= 0) & (Jy < gc.NY) & (Jz >= 0) &
+                   (Jz < gc.NZ);
+  const long nnf = gf.n_nodes, nnc = gc.n_nodes;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (Jin) {
+    for (int a = -1; a <= 1; ++a) {
+      const int fx = 2 * ix + a;
+      if (fx < 0 || fx >= gf.NX) continue;
+      const double wxv = pweight(fx, ix, gc.NX);
+      if (wxv == 0.0) continue;
+      const int qx = Jx - jbase(fx);
+      if (qx < 0 || qx > 2) continue;
+      for (int b = -1; b <= 1; ++b) {
+        const int fy = 2 * iy + b;
+        if (fy < 0 || fy >= gf.NY) continue;
+        const double wyv = pweight(fy, iy, gc.NY);
+        if (wyv == 0.0) continue;
+        const int qy = Jy - jbase(fy);
+        if (qy < 0 || qy > 2) continue;
+        for (int c = -1; c <= 1; ++c) {
+          const int fz = 2 * iz + c;
+          if (fz < 0 || fz >= gf.NZ) continue;
+          const double wzv = pweight(fz, iz, gc.NZ);
+          if (wzv == 0.0) continue;
+          const int qz = Jz - jbase(fz);
+          if (qz < 0 || qz > 2) continue;
+          const double w = (wxv * wyv) * wzv;
+          const int a_node = (fx * gf.NY + fy) * gf.NZ + fz;
+          const int q = qx * 9 + qy * 3 + qz;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            if (mapf(3 * a_node + r) < 0) continue;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              acc[r * 3 + c] += w * KP[(long)(q * 9 + r * 3 + c) * nnf + a_node];
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const bool rf = mapc(3 * n + r) >= 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      Ac[(long)(k * 9 + r * 3 + c) * nnc + n] = rf ? acc[r * 3 + c] : 0.0;
+  }
+}
+
+int rap(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc, const double* Af,
+        double* KP, double* Ac, hipStream_t st) {
+  hipLaunchKernelGGL(k_rap_kp, dim3(cdiv(gf.n_nodes, kBlock), 27), dim3(kBlock), 0, st, gf, mapf,
+                     gc, mapc, Af, KP);
+  GF_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_rap_r, dim3(cdiv(gc.n_nodes, kBlock), 27), dim3(kBlock), 0, st, gf, mapf,
+                     gc, mapc, KP, Ac);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// dense coarsest: gather, Cholesky (one work-group), two triangular solves
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_to_dense(Grid g, DofMap map,
+                                                     const double* __restrict__ A, int nf,
+                                                     double* __restrict__ D) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (n >= g.n_nodes) return;
+  int ix, iy, iz;
+  node_xyz(g, n, ix, iy, iz);
+  const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+  if (jx < 0 || jx >= g.NX || jy < 0 || jy >= g.NY || jz < 0 || jz >= g.NZ) return;
+  const int m = (jx * g.NY + jy) * g.NZ + jz;
+  const long nn = g.n_nodes;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int fr = map(3 * n + r);
+    if (fr < 0) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int fc = map(3 * m + c);
+      if (fc < 0) continue;
+      D[(long)fr * nf + fc] = A[(long)(k * 9 + r * 3 + c) * nn + n];
+    }
+  }
+}
+
+int stencil_to_dense(const Grid& g, const DofMap& map, const double* A, int n_free, double* D,
+                     hipStream_t st) {
+  GF_HIP(hipMemsetAsync(D, 0, sizeof(double) * (size_t)n_free * n_free, st));
+  hipLaunchKernelGGL(k_to_dense, dim3(cdiv(g.n_nodes, kBlock), 27), dim3(kBlock), 0, st, g, map,
+                     A, n_free, D);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+__global__ __launch_bounds__(1024) void k_cholesky(int n, double* __restrict__ D,
+                                                   double* __restrict__ Lt, int* fail) {
+  __shared__ double piv;
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (threadIdx.x == 0) {
+      const double d = D[(long)k * n + k];
+      if (!(d > 0.0) || !isfinite(d)) bad = 1;
+      piv = sqrt(d);
+      D[(long)k * n + k] = piv;
+    }
+    __syncthreads();
+    if (bad) break;
+    const double p = piv;
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) D[(long)i * n + k] /= p;
+    __syncthreads();
+    // trailing update of the lower triangle, rows distributed over threads
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      const double lik = D[(long)i * n + k];
+      double* row = D + (long)i * n;
+      const double* rowk = D;  // column k of rows j read below
+      for (int j = k + 1; j <= i; ++j) row[j] -= lik * rowk[(long)j * n + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *fail = bad;
+  __syncthreads();
+  if (bad) return;
+  // zero the strict upper triangle, write the transpose (column-major copy)
+  for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    if (j > i) D[idx] = 0.0;
+  }
+  __syncthreads();
+  for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    Lt[(long)j * n + i] = D[idx];
+  }
+}
+
+int dense_cholesky(int n, double* D, double* Lt, int* fail, hipStream_t st) {
+  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1024), 0, st, n, D, Lt, fail);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_diag(int n, double* D, double eps) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    D[(long)i * n + i] += eps;
+}
+
+int dense_add_diag(int n, double* D, double eps, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_diag, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, n, D, eps);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// x = L^-T L^-1 b, column sweeps; the working vector lives in LDS.
+constexpr int kTrsvThreads = 512;
+constexpr int kTrsvMaxN = 5000;
+
+__global__ __launch_bounds__(kTrsvThreads) void k_chol_solve(int n, const double* __restrict__ L,
+                                                             const double* __restrict__ Lt,
+                                                             const double* __restrict__ b,
+                                                             double* __restrict__ x,
+                                                             const int* gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double v[kTrsvMaxN];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = b[i];
+  __syncthreads();
+  // forward: L y = b   (column j of L is row j of Lt: contiguous)
+  for (int j = 0; j < n; ++j) {
+    const double yj = v[j] / Lt[(long)j * n + j];
+    __syncthreads();
+    if (threadIdx.x == 0) v[j] = yj;
+    const double* col = Lt + (long)j * n;
+    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) v[i] -= col[i] * yj;
+    __syncthreads();
+  }
+  // backward: L^T x = y  (column j of L^T is row j of L: contiguous)
+  for (int j = n - 1; j >= 0; --j) {
+    const double xj = v[j] / L[(long)j * n + j];
+    __syncthreads();
+    if (threadIdx.x == 0) v[j] = xj;
+    const double* row = L + (long)j * n;
+    for (int i = threadIdx.x; i < j; i += blockDim.x) v[i] -= row[i] * xj;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = v[i];
+}
+
+int dense_chol_solve(int n, const double* L, const double* Lt, const double* b, double* x,
+                     const int* gate, hipStream_t st) {
+  if (n > kTrsvMaxN) {
+    set_error("dense_chol_solve: n exceeds the dense coarsest limit");
+    return -2;
+  }
+  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(kTrsvThreads), 0, st, n, L, Lt, b, x, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// fixed-count Jacobi PCG on the coarsest stencil (one persistent work-group;
+// the reference's device-scalar loop, solver_v2.py:781-810)
+// ---------------------------------------------------------------------------
+constexpr int kPcgThreads = 1024;
+
+__device__ double wg_sum(double v, double* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
+    lds[16] = s;
+  }
+  __syncthreads();
+  return lds[16];
+}
+
+__global__ __launch_bounds__(kPcgThreads) void k_fixed_pcg(Grid g, DofMap map,
+                                                           const double* __restrict__ A,
+                                                           const double* __restrict__ dinv,
+                                                           int nf, int iters,
+                                                           const double* __restrict__ b,
+                                                           double* __restrict__ x,
+                                                           double* __restrict__ r,
+                                                           double* __restrict__ p,
+                                                           double* __restrict__ Ap,
+                                                           const int* gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double lds[17];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+    x[i] = 0.0;
+    r[i] = b[i];
+    const double z = dinv[i] * b[i];
+    p[i] = z;
+    acc += b[i] * z;
+  }
+  double rz = wg_sum(acc, lds);
+  const long nn = g.n_nodes;
+  for (int it = 0; it < iters; ++it) {
+    __syncthreads();
+    double pap = 0.0;
+    for (int n = threadIdx.x; n < g.n_nodes; n += blockDim.x) {
+      const int o[3] = {map(3 * n), map(3 * n + 1), map(3 * n + 2)};
+      if ((o[0] & o[1] & o[2]) < 0) continue;
+      const int iz = n % g.NZ, t = n / g.NZ, iy = t % g.NY, ix = t / g.NY;
+      double a0 = 0, a1 = 0, a2 = 0;
+      for (int k = 0; k < 27; ++k) {
+        const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+        if (jx < 0 || jx >= g.NX || jy < 0 || jy >= g.NY || jz < 0 || jz >= g.NZ) continue;
+        const int m = (jx * g.NY + jy) * g.NZ + jz;
+        for (int q = 0; q < 3; ++q) {
+          const int f = map(3 * m + q);
+          if (f < 0) continue;
+          const double pv = p[f];
+          a0 += A[(k * 9 + q) * nn + n] * pv;
+          a1 += A[(k * 9 + 3 + q) * nn + n] * pv;
+          a2 += A[(k * 9 + 6 + q) * nn + n] * pv;
+        }
+      }
+      const double av[3] = {a0, a1, a2};
+      for (int q = 0; q < 3; ++q)
+        if (o[q] >= 0) Ap[o[q]] = av[q];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) pap += p[i] * Ap[i];
+    const double pAp = wg_sum(pap, lds);
+    const double alpha = rz / (pAp + 1e-300);
+    double rzn = 0.0;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+      x[i] = x[i] + alpha * p[i];
+      const double ri = r[i] - alpha * Ap[i];
+      r[i] = ri;
+      rzn += ri * (dinv[i] * ri);
+    }
+    const double rz_n = wg_sum(rzn, lds);
+    const double beta = rz_n / (rz + 1e-300);
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) p[i] = dinv[i] * r[i] + beta * p[i];
+    rz = rz_n;
+  }
+}
+
+int fixed_pcg(const Grid& g, const DofMap& map, const double* A, const double* dinv, int n_free,
+              int iters, const double* b, double* x, double* work, const int* gate,
+              hipStream_t st) {
+  hipLaunchKernelGGL(k_fixed_pcg, dim3(1), dim3(kPcgThreads), 0, st, g, map, A, dinv, n_free,
+                     iters, b, x, work, work + n_free, work + 2L * n_free, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace gf

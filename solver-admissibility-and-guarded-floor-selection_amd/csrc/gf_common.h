@@ -1,0 +1,86 @@
+// Shared definitions for the gfhip library (gfx950 / CDNA4, wave64).
+//
+// Layout conventions (identical at every multigrid level):
+//   node (ix,iy,iz) of an (nx,ny,nz)-element grid has linear index
+//   n = (ix*(ny+1) + iy)*(nz+1) + iz, DOF = 3n + component.
+//   Vectors are indexed by *free* DOF (the reference's u_free layout); a
+//   DofMap turns a full DOF into its free index or -1.
+//   Coarse operators are 27-point 3x3-block stencils stored SoA:
+//   A[(k*9 + r*3 + c)*n_nodes + n], k = (dx+1)*9 + (dy+1)*3 + (dz+1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace gf {
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define GF_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t _e = (call);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      ::gf::set_error(std::string(#call) + ": " + hipGetErrorString(_e));     \
+      return -1;                                                              \
+    }                                                                         \
+  } while (0)
+
+#define GF_CHECK_LAUNCH()                                                     \
+  do {                                                                        \
+    hipError_t _e = hipGetLastError();                                        \
+    if (_e != hipSuccess) {                                                   \
+      ::gf::set_error(std::string("kernel launch: ") + hipGetErrorString(_e)); \
+      return -1;                                                              \
+    }                                                                         \
+  } while (0)
+
+#define GF_TRY(expr)                                                          \
+  do {                                                                        \
+    int _rc = (expr);                                                         \
+    if (_rc != 0) return _rc;                                                 \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 512;  // fixed reduction grid => deterministic sums
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Geometry of one level's node grid.
+struct Grid {
+  int nx, ny, nz;      // elements
+  int NX, NY, NZ;      // nodes = elements + 1
+  int n_nodes;
+  int ndof;
+};
+
+inline Grid make_grid(int nx, int ny, int nz) {
+  Grid g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.NX = nx + 1; g.NY = ny + 1; g.NZ = nz + 1;
+  g.n_nodes = g.NX * g.NY * g.NZ;
+  g.ndof = 3 * g.n_nodes;
+  return g;
+}
+
+// Full DOF -> free index.  Offset mode (off >= 0): the free DOFs are exactly
+// [off, ndof) -- true for any face clamp at x = 0 -- so the index is computed,
+// not loaded.  Table mode (off < 0): f2f[dof].
+struct DofMap {
+  int off;
+  const int* f2f;
+  __device__ __forceinline__ int operator()(int dof) const {
+    if (off >= 0) return dof >= off ? dof - off : -1;
+    return f2f[dof];
+  }
+};
+
+// Device scalar slots (FP64) used by device-resident control flow.
+struct Scalars {
+  double* s;   // base pointer of a device array of doubles
+  int* ticket; // reduction tickets (one per concurrent reduction site)
+};
+
+}  // namespace gf

@@ -1,0 +1,370 @@
+// Fine-level matrix-free hex8 operator K_ff(E) u and its set-up companions.
+//
+// Owner-computes formulation: one thread per grid node gathers the 27
+// neighbouring node displacements (3 components each) and the moduli of its
+// (up to) 8 adjacent elements into registers, then forms
+//     y_n = sum_{e adjacent} E_e * (Ke[rows of n in e, :] . u_e)
+// with the element loop in ascending element index and the 24-term row
+// products in column order.  No atomics, no element->DOF table (indices are
+// computed from the structured grid), and every result is bitwise
+// reproducible.  Ke is read with wave-uniform addresses, so it travels through
+// the scalar cache and feeds the FMAs as scalar operands.
+#include "gf_internal.h"
+
+namespace gf {
+
+// local node index of position (px,py,pz) in the reference hex ordering
+// (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1)
+__host__ __device__ constexpr int lidx(int px, int py, int pz) {
+  return pz * 4 + (py ? (px ? 2 : 3) : (px ? 1 : 0));
+}
+__host__ __device__ constexpr int lpos(int j, int axis) {
+  // inverse of lidx
+  return axis == 2 ? (j >> 2)
+                   : (axis == 1 ? (((j & 3) == 2 || (j & 3) == 3) ? 1 : 0)
+                                : (((j & 3) == 1 || (j & 3) == 2) ? 1 : 0));
+}
+
+template <typename T>
+struct Acc {
+  using type = T;
+};
+// FP32 operator: element products in FP32, node sums in FP64, result rounded
+// once to FP32 (the reference's FP32 path accumulates its scatter in FP64).
+template <>
+struct Acc<float> {
+  using type = double;
+};
+
+template <typename T, int EPI>
+__device__ __forceinline__ void apply_epilogue(int i, T ax, const T* __restrict__ x,
+                                               T* __restrict__ y, const EpiArgs<T>& ep) {
+  if (EPI == EPI_PLAIN) {
+    y[i] = ax;
+  } else if (EPI == EPI_RESID) {
+    y[i] = ep.b[i] - ax;
+  } else if (EPI == EPI_CHEB0) {
+    T z = ep.dinv[i] * (ep.b[i] - ax);
+    ep.dv[i] = z;
+    ep.xo[i] = x[i] + ep.a * z;
+  } else if (EPI == EPI_CHEB) {
+    T z = ep.dinv[i] * (ep.b[i] - ax);
+    T d = ep.a * (z + ep.c * ep.dv[i]);
+    ep.dv[i] = d;
+    ep.xo[i] = x[i] + d;
+  } else {  // JACOBI
+    ep.xo[i] = x[i] + (ep.a * ep.dinv[i]) * (ep.b[i] - ax);
+  }
+}
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(kBlock) void k_fine_matvec(Grid g, DofMap map,
+                                                        const T* __restrict__ Ke,
+                                                        const T* __restrict__ E,
+                                                        const T* __restrict__ u,
+                                                        T* __restrict__ y, EpiArgs<T> ep,
+                                                        const int* gate) {
+  using A = typename Acc<T>::type;
+  if (gate && *gate == 0) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int iz = n % g.NZ;
+  const int t = n / g.NZ;
+  const int iy = t % g.NY;
+  const int ix = t / g.NY;
+  const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
+  if ((o0 & o1 & o2) < 0) return;  // all three components fixed
+
+  T un[27][3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const int jx = ix + dx - 1, jy = iy + dy - 1, jz = iz + dz - 1;
+        const bool in = (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
+                        (jz < g.NZ);
+        const int m = (jx * g.NY + jy) * g.NZ + jz;
+        const int k = dx * 9 + dy * 3 + dz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int f = in ? map(3 * m + c) : -1;
+          un[k][c] = f >= 0 ? u[f] : T(0);
+        }
+      }
+  T Ee[8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int ex = ix - 1 + a, ey = iy - 1 + b, ez = iz - 1 + c;
+        const bool in = (ex >= 0) & (ex < g.nx) & (ey >= 0) & (ey < g.ny) & (ez >= 0) &
+                        (ez < g.nz);
+        Ee[a * 4 + b * 2 + c] = in ? E[(ex * g.ny + ey) * g.nz + ez] : T(0);
+      }
+
+  A acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int L = lidx(1 - a, 1 - b, 1 - c);
+        T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = (a + lpos(j, 0)) * 9 + (b + lpos(j, 1)) * 3 + (c + lpos(j, 2));
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const T uu = un[k][q];
+            t0 += Ke[(3 * L + 0) * 24 + 3 * j + q] * uu;
+            t1 += Ke[(3 * L + 1) * 24 + 3 * j + q] * uu;
+            t2 += Ke[(3 * L + 2) * 24 + 3 * j + q] * uu;
+          }
+        }
+        const T e = Ee[a * 4 + b * 2 + c];
+        acc0 += (A)(e * t0);
+        acc1 += (A)(e * t1);
+        acc2 += (A)(e * t2);
+      }
+  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)acc0, u, y, ep);
+  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)acc1, u, y, ep);
+  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)acc2, u, y, ep);
+}
+
+template <typename T>
+int fine_matvec(const Grid& g, const DofMap& map, const T* Ke, const T* E, const T* u, T* y,
+                int epi, const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
+  dim3 grid(cdiv(g.n_nodes, kBlock)), block(kBlock);
+  switch (epi) {
+    case EPI_PLAIN:
+      hipLaunchKernelGGL((k_fine_matvec<T, EPI_PLAIN>), grid, block, 0, st, g, map, Ke, E, u, y,
+                         ep, gate);
+      break;
+    case EPI_RESID:
+      hipLaunchKernelGGL((k_fine_matvec<T, EPI_RESID>), grid, block, 0, st, g, map, Ke, E, u, y,
+                         ep, gate);
+      break;
+    case EPI_CHEB0:
+      hipLaunchKernelGGL((k_fine_matvec<T, EPI_CHEB0>), grid, block, 0, st, g, map, Ke, E, u, y,
+                         ep, gate);
+      break;
+    case EPI_CHEB:
+      hipLaunchKernelGGL((k_fine_matvec<T, EPI_CHEB>), grid, block, 0, st, g, map, Ke, E, u, y,
+                         ep, gate);
+      break;
+    case EPI_JACOBI:
+      hipLaunchKernelGGL((k_fine_matvec<T, EPI_JACOBI>), grid, block, 0, st, g, map, Ke, E, u, y,
+                         ep, gate);
+      break;
+    default:
+      set_error("fine_matvec: bad epilogue");
+      return -2;
+  }
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+template int fine_matvec<double>(const Grid&, const DofMap&, const double*, const double*,
+                                 const double*, double*, int, const EpiArgs<double>&, const int*,
+                                 hipStream_t);
+template int fine_matvec<float>(const Grid&, const DofMap&, const float*, const float*,
+                                const float*, float*, int, const EpiArgs<float>&, const int*,
+                                hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_start(long n, const T* __restrict__ b,
+                                                       const T* __restrict__ dinv, T a,
+                                                       T* __restrict__ dv, T* __restrict__ x,
+                                                       const int* gate) {
+  if (gate && *gate == 0) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    if (dv) {
+      T z = dinv[i] * b[i];
+      dv[i] = z;
+      x[i] = T(0) + a * z;
+    } else {
+      x[i] = T(0) + (a * dinv[i]) * b[i];
+    }
+  }
+}
+
+template <typename T>
+int zero_start_step(long n, const T* b, const T* dinv, T a, T* dv, T* x, const int* gate,
+                    hipStream_t st) {
+  int grid = cdiv(n, kBlock);
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_zero_start<T>, dim3(grid), dim3(kBlock), 0, st, n, b, dinv, a, dv, x,
+                     gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+template int zero_start_step<double>(long, const double*, const double*, double, double*, double*,
+                                     const int*, hipStream_t);
+template int zero_start_step<float>(long, const float*, const float*, float, float*, float*,
+                                    const int*, hipStream_t);
+
+// diag(K_ff)_dof = sum_{e adjacent, ascending} E_e * Ke[jj, jj]
+__global__ __launch_bounds__(kBlock) void k_fine_diag(Grid g, DofMap map,
+                                                      const double* __restrict__ Ke,
+                                                      const double* __restrict__ E,
+                                                      double* __restrict__ diag) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int iz = n % g.NZ, t = n / g.NZ, iy = t % g.NY, ix = t / g.NY;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int ex = ix - 1 + a, ey = iy - 1 + b, ez = iz - 1 + c;
+        if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
+        const double e = E[(ex * g.ny + ey) * g.nz + ez];
+        const int L = lidx(1 - a, 1 - b, 1 - c);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] += e * Ke[(3 * L + q) * 25];
+      }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int f = map(3 * n + q);
+    if (f >= 0) diag[f] = acc[q];
+  }
+}
+
+int fine_diag(const Grid& g, const DofMap& map, const double* Ke, const double* E, double* diag,
+              hipStream_t st) {
+  hipLaunchKernelGGL(k_fine_diag, dim3(cdiv(g.n_nodes, kBlock)), dim3(kBlock), 0, st, g, map, Ke,
+                     E, diag);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_safe_inverse(long n, const double* __restrict__ d,
+                                                         double* __restrict__ dinv,
+                                                         float* __restrict__ dinv32) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double v = d[i];
+    const double s = fabs(v) > 1e-12 ? v : 1.0;
+    const double r = 1.0 / s;
+    dinv[i] = r;
+    if (dinv32) dinv32[i] = (float)r;
+  }
+}
+
+int safe_inverse(long n, const double* d, double* dinv, float* dinv32, hipStream_t st) {
+  int grid = cdiv(n, kBlock);
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_safe_inverse, dim3(grid), dim3(kBlock), 0, st, n, d, dinv, dinv32);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// Node-block Jacobi: 3x3 node block sum_e E_e Ke[3L:3L+3, 3L:3L+3] (ascending
+// elements), inverted by cofactors for nodes whose 3 DOFs are free; scalar
+// safe inverse of the supplied diagonal for partially constrained nodes.
+__global__ __launch_bounds__(kBlock) void k_nodeblock_update(Grid g, DofMap map,
+                                                             const double* __restrict__ Ke,
+                                                             const double* __restrict__ E,
+                                                             const double* __restrict__ sdiag,
+                                                             double* __restrict__ inv9,
+                                                             double* __restrict__ pinv) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int o[3] = {map(3 * n), map(3 * n + 1), map(3 * n + 2)};
+  const bool all = (o[0] >= 0) & (o[1] >= 0) & (o[2] >= 0);
+  const bool any = (o[0] >= 0) | (o[1] >= 0) | (o[2] >= 0);
+  if (!any) return;
+  if (!all) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (o[q] >= 0 && sdiag) {
+        const double v = sdiag[o[q]];
+        pinv[o[q]] = 1.0 / (fabs(v) > 1e-12 ? v : 1.0);
+      }
+    return;
+  }
+  const int iz = n % g.NZ, t = n / g.NZ, iy = t % g.NY, ix = t / g.NY;
+  double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int ex = ix - 1 + a, ey = iy - 1 + b, ez = iz - 1 + c;
+        if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
+        const double e = E[(ex * g.ny + ey) * g.nz + ez];
+        const int L = lidx(1 - a, 1 - b, 1 - c);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int s = 0; s < 3; ++s) B[r * 3 + s] += e * Ke[(3 * L + r) * 24 + 3 * L + s];
+      }
+  const double c00 = B[4] * B[8] - B[5] * B[7];
+  const double c01 = B[5] * B[6] - B[3] * B[8];
+  const double c02 = B[3] * B[7] - B[4] * B[6];
+  const double det = B[0] * c00 + B[1] * c01 + B[2] * c02;
+  const double id = 1.0 / det;
+  double* I = inv9 + 9L * n;
+  I[0] = c00 * id;
+  I[1] = (B[2] * B[7] - B[1] * B[8]) * id;
+  I[2] = (B[1] * B[5] - B[2] * B[4]) * id;
+  I[3] = c01 * id;
+  I[4] = (B[0] * B[8] - B[2] * B[6]) * id;
+  I[5] = (B[2] * B[3] - B[0] * B[5]) * id;
+  I[6] = c02 * id;
+  I[7] = (B[1] * B[6] - B[0] * B[7]) * id;
+  I[8] = (B[0] * B[4] - B[1] * B[3]) * id;
+}
+
+int nodeblock_update(const Grid& g, const DofMap& map, const double* Ke, const double* E,
+                     const double* scalar_diag, double* inv9, double* pinv, hipStream_t st) {
+  hipLaunchKernelGGL(k_nodeblock_update, dim3(cdiv(g.n_nodes, kBlock)), dim3(kBlock), 0, st, g,
+                     map, Ke, E, scalar_diag, inv9, pinv);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nodeblock_apply(Grid g, DofMap map,
+                                                            const double* __restrict__ inv9,
+                                                            const double* __restrict__ pinv,
+                                                            int has_pinv,
+                                                            const double* __restrict__ r,
+                                                            double* __restrict__ z,
+                                                            const int* gate) {
+  if (gate && *gate == 0) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
+  if ((o0 >= 0) & (o1 >= 0) & (o2 >= 0)) {
+    const double* I = inv9 + 9L * n;
+    const double r0 = r[o0], r1 = r[o1], r2 = r[o2];
+    z[o0] = I[0] * r0 + I[1] * r1 + I[2] * r2;
+    z[o1] = I[3] * r0 + I[4] * r1 + I[5] * r2;
+    z[o2] = I[6] * r0 + I[7] * r1 + I[8] * r2;
+    return;
+  }
+  const int o[3] = {o0, o1, o2};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (o[q] >= 0) z[o[q]] = has_pinv ? pinv[o[q]] * r[o[q]] : 0.0;
+}
+
+int nodeblock_apply(const Grid& g, const DofMap& map, const double* inv9, const double* pinv,
+                    int has_pinv, const double* r, double* z, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_nodeblock_apply, dim3(cdiv(g.n_nodes, kBlock)), dim3(kBlock), 0, st, g,
+                     map, inv9, pinv, has_pinv, r, z, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace gf

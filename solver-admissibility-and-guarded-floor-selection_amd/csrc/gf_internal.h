@@ -1,0 +1,103 @@
+// Internal interfaces between the gfhip translation units.
+#pragma once
+
+#include "gf_common.h"
+
+#include <vector>
+
+namespace gf {
+
+// Reduction scratch shared by the kernels of one stream (fixed-size, owned
+// by the object that issues the launches).
+struct Work {
+  double* partials = nullptr;  // 4 * kRedBlocks doubles
+  int* ticket = nullptr;
+  int init();
+  void release();
+};
+
+// Epilogue applied by the operator kernels to each output DOF i:
+//   PLAIN : y_i = (A x)_i
+//   RESID : y_i = b_i - (A x)_i
+//   CHEB0 : dv_i = dinv_i (b_i - (A x)_i);         xo_i = x_i + a dv_i
+//   CHEB  : dv_i = a (dinv_i (b_i - (A x)_i) + c dv_i); xo_i = x_i + dv_i
+//   JACOBI: xo_i = x_i + (a dinv_i) (b_i - (A x)_i)
+enum Epilogue { EPI_PLAIN = 0, EPI_RESID = 1, EPI_CHEB0 = 2, EPI_CHEB = 3, EPI_JACOBI = 4 };
+
+template <typename T>
+struct EpiArgs {
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  T* dv = nullptr;
+  T* xo = nullptr;
+  T a = 0;
+  T c = 0;
+};
+
+// ---- BLAS-1 (gf_blas.hip) --------------------------------------------------
+int dot_f64(Work& w, long n, const double* x, const double* y, double* out, bool sqrt_,
+            const int* gate, hipStream_t st);
+int dot_f32(Work& w, long n, const float* x, const float* y, double* out, bool sqrt_,
+            const int* gate, hipStream_t st);
+int dot4_ls(Work& w, long n, const double* c, const double* r, const double* Ac, double* out4,
+            const int* gate, hipStream_t st);
+int mgs_axpy_dot(Work& w, long n, const double* v, double* wv, const double* h,
+                 const double* next, double* out, bool sqrt_, const int* gate, hipStream_t st);
+int trial_norm(Work& w, long n, const double* b, const double* x, const double* s, double* y,
+               double* out, const int* gate, hipStream_t st);
+int axpby_dev(long n, const double* a, double asign, const double* x, const double* b, double* y,
+              const int* gate, hipStream_t st);
+int scale_inv(long n, const double* x, const double* h, double thresh, double* y,
+              const int* gate, hipStream_t st);
+int combine(long n, const double* Z, long ldz, const double* y, const int* jcount, bool acc,
+            double* x, const int* gate, hipStream_t st);
+int cast_f64_to_f32(long n, const double* x, float* y, hipStream_t st);
+int cast_f32_to_f64(long n, const float* x, double* y, hipStream_t st);
+int sub_f64(long n, const double* a, const double* b, double* y, hipStream_t st);
+int minmax_f64(Work& w, long n, const double* E, double* out2, hipStream_t st);
+int count_gt(Work& w, long n, const double* E, const double* thr_src, double scale, double* out,
+             hipStream_t st);
+
+// ---- fine level (gf_fine.hip) ----------------------------------------------
+// y = K_ff(E) u over the structured hex grid `g` (free layout via `map`).
+template <typename T>
+int fine_matvec(const Grid& g, const DofMap& map, const T* Ke, const T* E, const T* u, T* y,
+                int epi, const EpiArgs<T>& ep, const int* gate, hipStream_t st);
+// Zero-start first Chebyshev/Jacobi step: dv = dinv*b; x = a*dv (dv may be null).
+template <typename T>
+int zero_start_step(long n, const T* b, const T* dinv, T a, T* dv, T* x, const int* gate,
+                    hipStream_t st);
+int fine_diag(const Grid& g, const DofMap& map, const double* Ke, const double* E, double* diag,
+              hipStream_t st);
+// diag -> 1/where(|d|>1e-12, d, 1) (also fp32 copy when d32 != null)
+int safe_inverse(long n, const double* d, double* dinv, float* dinv32, hipStream_t st);
+int nodeblock_update(const Grid& g, const DofMap& map, const double* Ke, const double* E,
+                     const double* scalar_diag, double* inv9, double* pinv, hipStream_t st);
+int nodeblock_apply(const Grid& g, const DofMap& map, const double* inv9, const double* pinv,
+                    int has_pinv, const double* r, double* z, const int* gate, hipStream_t st);
+
+// ---- coarse levels (gf_coarse.hip) -----------------------------------------
+int level1_assemble(const Grid& gc, const DofMap& mapc, const Grid& gf, const double* Q,
+                    const double* E, double* A, hipStream_t st);
+template <typename T>
+int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
+                   const EpiArgs<T>& ep, const int* gate, hipStream_t st);
+int stencil_diag(const Grid& g, const DofMap& map, const double* A, double* diag, hipStream_t st);
+int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
+            const double* ec, double* yf, const int* gate, hipStream_t st);
+int restrict_(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
+              const double* rf, double* yc, const int* gate, hipStream_t st);
+int rap(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
+        const double* Af, double* KP, double* Ac, hipStream_t st);
+int stencil_to_dense(const Grid& g, const DofMap& map, const double* A, int n_free, double* D,
+                     hipStream_t st);
+int dense_cholesky(int n, double* D, double* Lt, int* fail, hipStream_t st);
+int dense_add_diag(int n, double* D, double eps, hipStream_t st);
+int dense_chol_solve(int n, const double* L, const double* Lt, const double* b, double* x,
+                     const int* gate, hipStream_t st);
+int fixed_pcg(const Grid& g, const DofMap& map, const double* A, const double* dinv,
+              int n_free, int iters, const double* b, double* x, double* work,
+              const int* gate, hipStream_t st);
+int f64_to_f32_block(long n, const double* A, float* A32, hipStream_t st);
+
+}  // namespace gf

@@ -1,0 +1,146 @@
+// Solver objects behind the C ABI: the fine-operator context, the Galerkin
+// GMG hierarchy with its nonlinear wrapper, and device-resident FGMRES.
+#pragma once
+
+#include "gf_internal.h"
+#include "../../include/gfhip.h"
+
+#include <functional>
+#include <vector>
+
+namespace gf {
+
+// Fine operator: structured hex grid, free-DOF layout, unit stiffness.
+struct Ctx {
+  Grid g{};
+  int n_free = 0;
+  int off = -1;                 // offset-map start or -1 (table mode)
+  int* d_f2f = nullptr;         // ndof ints
+  std::vector<int> h_free;
+  std::vector<int> h_f2f;
+  double h_Ke[576];
+  double* Ke = nullptr;
+  float* Ke32 = nullptr;
+  Work work;
+  // node-block Jacobi state (solver_v2.py NodeBlockJacobiMF)
+  double* nb_inv9 = nullptr;
+  double* nb_pinv = nullptr;
+  int nb_has_pinv = 0;
+  DofMap map() const { return DofMap{off, d_f2f}; }
+  int init(int nelx, int nely, int nelz, const double* Ke_host, const int* free_host, int nfree,
+           int ndof);
+  int nodeblock_alloc();
+  void release();
+};
+
+// One device-resident flexible GMRES instance (fixed maximum restart).
+struct Krylov {
+  int m = 0;
+  long n = 0;
+  double* V = nullptr;   // (m+1) x n
+  double* Z = nullptr;   // m x n
+  double* H = nullptr;   // (m+1) x m, row-major
+  double* cs = nullptr;
+  double* sn = nullptr;
+  double* g = nullptr;   // m+1
+  double* y = nullptr;   // m
+  double* sc = nullptr;  // scalars: [0]=bnorm [1]=beta [2]=hist value
+  int* fl = nullptr;     // [0]=step gate [1]=run gate [2]=jfinal [3]=converged [4]=total [5]=err
+  int alloc(int m_, long n_);
+  void release();
+};
+
+struct Level {
+  Grid g{};
+  int n_free = 0;
+  int off = -1;
+  int* d_f2f = nullptr;
+  std::vector<int> h_free;
+  double* A = nullptr;     // 27x9 block stencil (lv >= 1)
+  float* A32 = nullptr;
+  double* KP = nullptr;    // R(KP) scratch on the fine side of this level's product
+  double* diag = nullptr;
+  double* dinv = nullptr;
+  float* dinv32 = nullptr;
+  int prec = 0;            // 0 fp64, 1 fp32
+  // work vectors
+  double* X[2] = {nullptr, nullptr};
+  double *RHS = nullptr, *R = nullptr, *PE = nullptr, *AC = nullptr, *DV = nullptr,
+         *T = nullptr, *R2 = nullptr;
+  float *X32[2] = {nullptr, nullptr};
+  float *RHS32 = nullptr, *DV32 = nullptr;
+  double* st = nullptr;    // 16 device scalars for the safeguarded correction
+  DofMap map() const { return DofMap{off, d_f2f}; }
+};
+
+struct Gmg {
+  Ctx* ctx = nullptr;
+  gf_gmg_params p{};
+  int L = 1;
+  std::vector<Level> lv;
+  // set-up state
+  double* E64 = nullptr;
+  float* E32 = nullptr;
+  double* Q = nullptr;        // 8 x 576 sub-element Galerkin blocks
+  bool uploaded = false;
+  double lam0 = NAN;          // fine lambda_max estimate
+  double lam_E_max = NAN;
+  std::vector<double> lam;    // per level
+  double omega_used = 0.5;
+  double contrast = NAN, void_frac = NAN, E_max = NAN, E_min = NAN;
+  int high = 0, fine_deg = 2, coarse_it = 2;
+  int chol_mode = 0;          // 0 none, 1 dense, 2 iterative
+  int chol_regularised = 0;
+  double* CH = nullptr;       // dense L (row-major)
+  double* CHt = nullptr;      // its transpose
+  int ch_n = 0;
+  int* d_fail = nullptr;
+  double* pcg_work = nullptr;
+  double* re_w = nullptr;     // root-enrichment weights
+  double* nb_inv9 = nullptr;  // root-local node-block Jacobi (own copy)
+  double* nb_pinv = nullptr;
+  bool is_setup = false;
+  int setup_count = 0;
+  // wrapper work
+  Krylov ik, rl;
+  double *ap_x = nullptr, *ap_res = nullptr, *ap_corr = nullptr, *ap_trial = nullptr,
+         *ap_st = nullptr;
+  // outer solve
+  Krylov outer;
+  double* hs = nullptr;       // pinned host mirror for outer control
+  // scratch for power iterations
+  double *pw_v = nullptr, *pw_d = nullptr, *pw_kv = nullptr, *pw_av = nullptr, *pw_s = nullptr;
+  Work work;
+
+  int create(Ctx* c, const gf_gmg_params* params);
+  int setup(const double* E, const double* fine_diag, hipStream_t st);
+  int apply(const double* rhs, double* out, hipStream_t st);
+  int base_cycle(const double* rhs, double* out, const int* gate, hipStream_t st);
+  int outer_solve(const double* b, double* x, int x0_given, double tol, int maxiter, int restart,
+                  double* history, int hist_cap, int* n_hist, int* iters, int* converged,
+                  hipStream_t st);
+  void release();
+
+  // internals
+  int level_op(int l, const double* x, double* y, int epi, const EpiArgs<double>& ep,
+               const int* gate, hipStream_t st);
+  int smooth(int l, const double* x_in, const double* rhs, double* out, int sweeps_scale,
+             const int* gate, hipStream_t st, double** result);
+  int cheb(int l, const double* x_in, const double* rhs, double* out, const int* gate,
+           hipStream_t st, double** result);
+  int cycle(int l, const double* rhs, double* out, const int* gate, hipStream_t st,
+            double** result);
+  int correct(int l, double* x, const double* residual, const double* corr, double* residual_out,
+              const int* gate, hipStream_t st);
+  int solve_coarsest(const double* rhs, double* out, const int* gate, hipStream_t st,
+                     double** result);
+  int enrichment_rhs(const double* residual, double* out, const int* gate, hipStream_t st);
+  int fgmres_dev(Krylov& K, const std::function<int(const double*, double*, const int*)>& M,
+                 const double* b, double* x, double tol, int maxiter, int restart,
+                 hipStream_t st);
+  int estimate_fine_lambda(hipStream_t st, double* out);
+  int estimate_level_lambda(int l, hipStream_t st, double* out);
+  int read_scalars(const double* dptr, double* host, int n, hipStream_t st);
+};
+
+}  // namespace gf

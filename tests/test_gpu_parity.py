@@ -1,0 +1,176 @@
+"""HIP path vs the CPU oracle, through the C ABI (run on the MI355X box).
+
+Tolerances: FP64 operators 1e-13 relative (only summation order differs),
+FP32 operators 2e-5 relative to the FP64 result, the nonlinear preconditioner
+1e-10 relative, end-to-end policy decisions and iteration counts exact with
+compliance / true residual within 1e-10 relative.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STACK = dict(n_levels=4, fine_smoother="fp64", coarse_correction_policy="residual_line_search",
+             root_local_correction_mode="node_block_inner_fgmres",
+             root_local_correction_inner_steps=20, root_local_correction_inner_restart=20,
+             inner_krylov_steps=10, inner_krylov_restart=10)
+
+
+def _t(a):
+    import torch
+
+    return torch.as_tensor(np.asarray(a), device="cuda")
+
+
+def _n(t):
+    return t.detach().cpu().numpy()
+
+
+def _setup(dims, kind="tip", seed=0, p=0.35, rho_min=1e-12, penal=4.5, **stack):
+    import torch
+    from gpu_fem import policy as gpol
+    from oracle import fem, gmg as ogmg, policy as opol
+
+    if kind == "bridge":
+        spec_o = fem._bridge(dims, "b", (0.0, 0.25, 0.5))
+        from gpu_fem.presets import _bridge
+        spec_g = _bridge(*dims, (0.0, 0.25, 0.5))
+    else:
+        spec_o = fem.cantilever_spec(*dims, load=kind)
+        from gpu_fem.presets import cantilever
+        spec_g = cantilever(*dims, load=kind)
+    bc, free, edof, F, mf = fem.build_components(spec_o)
+    st = dict(STACK)
+    st.update(stack)
+    og = ogmg.GalerkinMatFreeGMG(mf_op=mf, free=free, nelx=dims[0], nely=dims[1], nelz=dims[2],
+                                 KE_UNIT=fem.KE_UNIT_3D, **st)
+    comps = gpol._build_components(spec_g, **st)
+    mask = opol.bernoulli_mask(seed, dims, p).ravel()
+    E = opol.simp_modulus(np.where(mask, 1.0, rho_min), rho_min, penal)
+    return dict(bc=bc, free=free, edof=edof, F=F, mf=mf, og=og, comps=comps, E=E, Et=_t(E),
+                torch=torch)
+
+
+@pytest.mark.parametrize("dims,kind", [((8, 4, 4), "tip"), ((9, 5, 3), "tip"),
+                                       ((12, 6, 6), "bridge"), ((20, 10, 10), "edge")])
+def test_fine_operator_parity(dims, kind):
+    s = _setup(dims, kind)
+    mf_g = s["comps"][6]
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(len(s["free"]))
+    y_o = s["mf"].matvec(u, s["E"])
+    y_g = _n(mf_g.matvec(_t(u), s["Et"]))
+    assert np.abs(y_g - y_o).max() <= 1e-13 * np.abs(y_o).max()
+    d_o = s["mf"].extract_diagonal(s["E"])
+    assert np.abs(_n(mf_g.extract_diagonal(s["Et"])) - d_o).max() <= 1e-14 * np.abs(d_o).max()
+    y32 = _n(mf_g.matvec(_t(u.astype(np.float32)), _t(s["E"].astype(np.float32))))
+    assert y32.dtype == np.float32
+    assert np.abs(y32 - y_o).max() <= 2e-5 * np.abs(y_o).max()
+    r = _n(mf_g.residual(_t(s["F"]), _t(u), s["Et"]))
+    assert np.abs(r - (s["F"] - y_o)).max() <= 1e-13 * np.abs(y_o).max()
+
+
+def test_nodeblock_parity():
+    from gpu_fem.solver_v2 import NodeBlockJacobiMF as NBg
+    from oracle.fem import NodeBlockJacobiMF as NBo, KE_UNIT_3D
+
+    s = _setup((10, 5, 5), "bridge")
+    free, ndof = s["free"], s["bc"].ndof
+    d = s["mf"].extract_diagonal(s["E"])
+    nbo = NBo(s["edof"], KE_UNIT_3D, free, len(free), ndof)
+    nbo.update(s["E"], scalar_diag=d)
+    nbg = NBg(_t(s["edof"]), _t(KE_UNIT_3D), _t(free), len(free), ndof)
+    nbg.update(s["Et"], scalar_diag=_t(d))
+    r = np.random.default_rng(9).standard_normal(len(free))
+    zo = nbo.apply(r)
+    zg = _n(nbg.apply(_t(r)))
+    assert np.abs(zg - zo).max() <= 1e-11 * np.abs(zo).max()
+
+
+@pytest.mark.parametrize("dims,kind,rho_min", [((16, 8, 8), "tip", 1e-12), ((15, 7, 7), "tip", 1e-3),
+                                               ((12, 6, 6), "bridge", 1e-2)])
+def test_hierarchy_setup_parity(dims, kind, rho_min):
+    s = _setup(dims, kind, rho_min=rho_min)
+    og, gg = s["og"], s["comps"][7]
+    og.setup(s["E"])
+    gg.setup(s["Et"])
+    assert og.adaptive_params["is_high_contrast"] == gg.is_high_contrast
+    assert og.adaptive_params["fine_degree_adapt"] == gg.adaptive_params["fine_degree_adapt"]
+    assert og.adaptive_params["coarse_iters_adapt"] == gg.adaptive_params["coarse_iters_adapt"]
+    assert abs(og.contrast_ratio - gg.contrast_ratio) <= 1e-12 * og.contrast_ratio
+    for a, b in zip(og.lambda_max_per_level, gg.lambda_max_per_level):
+        assert abs(a - b) <= 1e-10 * abs(a)
+    for lv in range(1, og._n_levels):
+        assert np.array_equal(og._free[lv], gg.level_free(lv))
+        Ko = og._K[lv].toarray()
+        Kg = _n(gg.level_dense(lv))
+        assert np.abs(Kg - Ko).max() <= 1e-12 * np.abs(Ko).max(), lv
+        assert np.allclose(_n(gg.level_dinv(lv)), og._d_inv[lv], rtol=1e-12, atol=0)
+    # transfers
+    for lv in range(og._n_levels - 1):
+        r = np.random.default_rng(lv).standard_normal(og._n_free[lv])
+        assert np.allclose(_n(gg.restrict(lv, _t(r))), og._R[lv] @ r, rtol=1e-13, atol=1e-14)
+        e = np.random.default_rng(lv + 7).standard_normal(og._n_free[lv + 1])
+        assert np.allclose(_n(gg.prolong(lv, _t(e))), og._P[lv] @ e, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("stack", [dict(), dict(coarse_correction_policy="none"),
+                                   dict(root_local_correction_mode="node_block_line_search"),
+                                   dict(cycle_type="w", inner_krylov_steps=0),
+                                   dict(smoother_type="jacobi"),
+                                   dict(root_enrichment_mode="weighted_gap_second"),
+                                   dict(fine_smoother="fp32",
+                                        level_precisions=["fp32", "fp32", "fp64", "fp64"])])
+def test_preconditioner_parity(stack):
+    s = _setup((16, 8, 8), "tip", rho_min=1e-3, **stack)
+    og, gg = s["og"], s["comps"][7]
+    og.setup(s["E"])
+    gg.setup(s["Et"])
+    rhs = np.random.default_rng(11).standard_normal(len(s["free"]))
+    zo = og.apply(rhs)
+    zg = _n(gg.apply(_t(rhs)))
+    tol = 1e-10 if stack.get("fine_smoother", "fp64") == "fp64" else 1e-4
+    assert np.abs(zg - zo).max() <= tol * np.abs(zo).max()
+    vo = og._apply_base_cycle(rhs)
+    vg = _n(gg.vcycle(_t(rhs)))
+    assert np.abs(vg - vo).max() <= tol * np.abs(vo).max()
+
+
+@pytest.mark.parametrize("dims,seed,p", [((16, 8, 8), 43, 0.35), ((16, 8, 8), 41, 0.15),
+                                         ((12, 6, 6), 7, 0.5)])
+def test_fgmres_solve_parity(dims, seed, p):
+    from oracle import krylov as okry
+
+    s = _setup(dims, "tip", seed=seed, p=p)
+    og, gg = s["og"], s["comps"][7]
+    og.setup(s["E"])
+    gg.setup(s["Et"])
+    ho, hg = [], []
+    xo, io_, co = okry.fgmres(lambda x: s["mf"].matvec(x, s["E"]), s["F"], og.apply, tol=1e-6,
+                              maxiter=100, restart=300, history=ho)
+    xg, ig, cg = gg.solve(_t(s["F"]), tol=1e-6, maxiter=100, restart=300, history=hg)
+    assert (io_, co) == (ig, cg)
+    assert len(ho) == len(hg)
+    h1, h2 = np.array(ho), np.array(hg)
+    assert np.all(np.abs(h1 - h2) <= 1e-8 * h1 + 1e-14)
+    comp_o = float(s["F"] @ xo)
+    comp_g = float(s["F"] @ _n(xg))
+    assert abs(comp_g - comp_o) <= 1e-10 * abs(comp_o)
+
+
+def test_generic_fgmres_matches_native():
+    from gpu_fem.multigrid_v4 import _cupy_fgmres
+
+    s = _setup((12, 6, 6), "tip", seed=3, p=0.4, rho_min=1e-3)
+    gg, mf_g = s["comps"][7], s["comps"][6]
+    gg.setup(s["Et"])
+    F = s["comps"][5]
+    h1, h2 = [], []
+    x1, i1, c1 = _cupy_fgmres(lambda x: mf_g.matvec(x, s["Et"]), F, lambda v: gg.apply(v),
+                              tol=1e-6, maxiter=60, restart=20, history=h1)
+    x2, i2, c2 = _cupy_fgmres(mf_g.operator(s["Et"]), F, gg.apply, tol=1e-6, maxiter=60,
+                              restart=20, history=h2)
+    assert (i1, c1) == (i2, c2)
+    assert np.allclose(h1, h2, rtol=1e-8, atol=1e-14)
+    assert np.allclose(_n(x1), _n(x2), rtol=1e-8, atol=1e-12)

@@ -1,0 +1,93 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every
+symbol include/gfhip.h declares; the product's host set-up (stiffness, DOF
+table, presets, boundary conditions, classification, connectivity) equals the
+oracle's."""
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "gfhip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gf_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from gpu_fem import _lib
+
+    lib = _lib.load()
+    names = _declared_symbols()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
+    assert lib.gf_version() >= 10000
+
+
+def test_params_struct_layout_matches_header():
+    import ctypes as C
+
+    from gpu_fem import _lib
+
+    lib = _lib.load()
+    p = _lib.gf_gmg_params()
+    lib.gf_gmg_default_params(C.byref(p))
+    assert p.n_levels == 4 and p.coarse_smooth_iters == 2 and p.dense_chol_max == 5000
+    assert abs(p.cheb_lower_frac - 1 / 30) < 1e-15 and p.coarse_pcg_iters == 80
+    assert list(p.level_prec) == [-1] * 8 and p.root_enrichment_weight_floor == 1e-6
+
+
+def test_library_reports_argument_errors_without_gpu():
+    import ctypes as C
+
+    from gpu_fem import _lib
+
+    lib = _lib.load()
+    h = C.c_void_p()
+    free = np.arange(10, dtype=np.int32)
+    ke = np.zeros(576)
+    rc = lib.gf_ctx_create(C.byref(h), 2, 2, 2, ke.ctypes.data, free.ctypes.data, 10, 12345)
+    assert rc == -2
+    assert b"ndof" in lib.gf_last_error()
+
+
+def test_host_setup_equals_oracle():
+    from gpu_fem import bc_generator, presets
+    from gpu_fem.pub_simp_solver import KE_UNIT_3D, _edof_table_3d, grid_from_edof
+    from oracle import fem
+
+    assert np.allclose(KE_UNIT_3D, fem.KE_UNIT_3D, rtol=0, atol=2e-16)
+    for dims in [(3, 2, 2), (8, 4, 5)]:
+        ed = _edof_table_3d(*dims)
+        assert np.array_equal(ed, fem.edof_hex8(*dims))
+        assert grid_from_edof(ed) == dims
+    with pytest.raises(ValueError):
+        grid_from_edof(_edof_table_3d(3, 2, 2)[::-1])
+    for name in ["cantilever_gpu_medium", "bridge_gpu_medium", "cantilever_3d"]:
+        a = bc_generator.generate_bc(presets.get_preset(name))
+        b = fem.generate_bc(fem.get_preset(name))
+        assert np.array_equal(a.free_dofs, b.free_dofs)
+        assert np.array_equal(a.F, b.F)
+    a = bc_generator.generate_bc(presets.cantilever(40, 20, 20, load="edge"))
+    b = fem.generate_bc(fem.cantilever_spec(40, 20, 20, load="edge"))
+    assert np.array_equal(a.F, b.F) and np.array_equal(a.free_dofs, b.free_dofs)
+
+
+def test_classify_and_connectivity_equal_oracle():
+    from gpu_fem.policy import _classify, _connectivity
+    from oracle import policy as opol
+
+    thr = dict(high_residual_threshold=1e-2, plateau_residual_threshold=1e-4,
+               plateau_ratio_threshold=0.6)
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        r50, r100 = 10 ** rng.uniform(-8, 0), 10 ** rng.uniform(-8, 0)
+        assert _classify(r50, r100, **thr) == opol.classify(r50, r100, **thr)
+    m = opol.bernoulli_mask(5, (20, 10, 10), 0.3)
+    assert _connectivity(m) == opol.connectivity(m)
