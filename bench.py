@@ -1,0 +1,243 @@
+"""Benchmark: accepted guarded solves/s of the plain FP64 GMG-FGMRES policy.
+
+One step = one density state taken through probe -> classify -> floor ladder
+-> recomputed-residual guard on one MI355X (the unit of the reference's
+headline number, recorded_policy_time_s).  Workload (default): cantilever
+80x40x20 (64,000 hex8 elements, 206,640 free DOFs), E0=1, nu=0.3, SIMP p=3,
+x=0 face clamped, unit downward load split over the x=L, y=0 edge nodes;
+Bernoulli densities p in {0.2, 0.35, 0.5} over seeds 0..31 (taken in
+seed-major order), ladder {1e-12 probe, 1e-3, 1e-2}, probe 100 / r50 at 50,
+thresholds 1e-2 / 1e-4 / 0.6, FGMRES restart=maxiter=300, tol 1e-6.
+
+Prints ONE JSON line (the driver contract) with a roofline object for the
+fine matvec (timed live with HIP events on the solver's stream) and a CPU
+baseline (the NumPy oracle on a bounded sample, scaled per state).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "solver-admissibility-and-guarded-floor-selection_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=9)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", default="80x40x20")
+    ap.add_argument("--penal", type=float, default=3.0)
+    ap.add_argument("--load", default="edge", choices=["edge", "tip"])
+    ap.add_argument("--probabilities", default="0.2,0.35,0.5")
+    ap.add_argument("--seeds", default=",".join(str(s) for s in range(32)))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--states-csv", default="")
+    return ap.parse_args()
+
+
+def states(args):
+    seeds = [int(s) for s in args.seeds.split(",") if s.strip()]
+    probs = [float(p) for p in args.probabilities.split(",") if p.strip()]
+    while True:
+        for s in seeds:
+            for p in probs:
+                yield s, p
+
+
+def matvec_roofline(runner, n_launch=200):
+    """Average duration of the FP64 fine matvec kernel on the solver's stream
+    (HIP events), and its algorithmic bytes / flops per launch."""
+    import torch
+
+    from gpu_fem import _lib
+
+    mf = runner.mf_op
+    n_free = mf._n_free
+    n_elem = mf._n_elem
+    E = torch.rand(n_elem, dtype=torch.float64, device="cuda") + 1e-3
+    u = torch.randn(n_free, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(u)
+    st = torch.cuda.current_stream()
+    args = (mf.ctx_handle, _lib.ptr(E), _lib.ptr(u), _lib.ptr(y), st.cuda_stream)
+    lib = _lib.load()
+    for _ in range(10):
+        lib.gf_matvec_f64(*args)
+    st.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(st)
+    for _ in range(n_launch):
+        lib.gf_matvec_f64(*args)
+    t1.record(st)
+    st.synchronize()
+    avg_us = t0.elapsed_time(t1) * 1e3 / n_launch
+    bytes_alg = 8 * n_elem + 2 * 8 * n_free
+    flops = 1152 * n_elem
+    gbs = bytes_alg / (avg_us * 1e-6) / 1e9
+    tfs = flops / (avg_us * 1e-6) / 1e12
+    return avg_us, bytes_alg, flops, gbs, tfs
+
+
+def cpu_baseline(args, dims, gpu_rows):
+    """The NumPy oracle on the box's host cores: one set-up and two canonical
+    FGMRES iterations on the first benchmark state at the probe floor
+    (about 20-40 s), scaled by the GPU run's mean iterations and set-ups per
+    accepted state.  Returns the JSON object."""
+    import numpy as np
+
+    from oracle import fem, gmg as ogmg, krylov as okry, policy as opol
+
+    spec = fem.cantilever_spec(*dims, load=args.load)
+    bc, free, edof, F, mf = fem.build_components(spec)
+    stack = dict(n_levels=4, fine_smoother="fp64", coarse_correction_policy="residual_line_search",
+                 root_local_correction_mode="node_block_inner_fgmres",
+                 root_local_correction_inner_steps=20, root_local_correction_inner_restart=20,
+                 inner_krylov_steps=10, inner_krylov_restart=10)
+    g = ogmg.GalerkinMatFreeGMG(mf_op=mf, free=free, nelx=dims[0], nely=dims[1], nelz=dims[2],
+                                KE_UNIT=fem.KE_UNIT_3D, **stack)
+    seed, p = gpu_rows[0]["seed"], gpu_rows[0]["solid_probability"]
+    mask = opol.bernoulli_mask(seed, dims, p).ravel()
+    E = opol.simp_modulus(np.where(mask, 1.0, 1e-12), 1e-12, args.penal)
+    t0 = time.perf_counter()
+    g.setup(E)
+    t_setup = time.perf_counter() - t0
+    # per-iteration cost of the canonical outer FGMRES on this state: time
+    # two outer iterations (the sample), scale to the GPU run's per-state work
+    t1 = time.perf_counter()
+    okry.fgmres(lambda x: mf.matvec(x, E), F, g.apply, tol=0.0, maxiter=2, restart=300)
+    per_iter = (time.perf_counter() - t1) / 2.0
+    acc = [r for r in gpu_rows if r["solve_converged"]]
+    it_per_state = float(np.mean([r["total_iters"] for r in acc])) if acc else float("nan")
+    setups_per_state = float(np.mean([r["n_setups"] for r in acc])) if acc else float("nan")
+    sec_per_state = it_per_state * per_iter + setups_per_state * t_setup
+    try:
+        import threadpoolctl
+
+        info = threadpoolctl.threadpool_info()
+        cores = max([i.get("num_threads", 1) for i in info] + [1])
+    except Exception:
+        cores = os.cpu_count() or 1
+    return {
+        "value": 1.0 / sec_per_state if sec_per_state > 0 else None,
+        "unit": "accepted guarded solves/s",
+        "cores": int(cores),
+        "kind": "port",
+        "sample": (f"oracle (NumPy/SciPy) on state seed={seed} p={p} {dims[0]}x{dims[1]}x{dims[2]}: "
+                   f"set-up {t_setup:.2f} s, one canonical FGMRES iteration {per_iter:.2f} s "
+                   f"(measured), scaled by the GPU run's {it_per_state:.1f} iterations and "
+                   f"{setups_per_state:.2f} set-ups per accepted state"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(states shard across GPUs with no communication; "
+                                   "multi-GPU runs are not part of this build)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from gpu_fem import policy as gpol
+
+    dims = tuple(int(v) for v in args.grid.split("x"))
+    runner = gpol.GuardedFloorPolicy.from_cantilever(
+        dims, load=args.load, penal=args.penal, raised_rho_mins=(1e-3, 1e-2), probe_iters=100,
+        probe_r50=50, restart=300, maxiter=300, tol=1e-6)
+    gen = states(args)
+
+    def one():
+        seed, p = next(gen)
+        mask = np.random.default_rng(seed).random(dims[0] * dims[1] * dims[2]) < p
+        row = runner.run_state(mask, seed=seed, probability=p, keep_x=False)
+        row["total_iters"] = (len(row["probe_history"]) - 1 +
+                              sum(len(h) - 1 for _, h in row["solve_history"]))
+        row["n_setups"] = 1 + row["n_floor_attempts"]
+        return row
+
+    for _ in range(args.warmup):
+        one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows = [one() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    accepted = sum(int(r["solve_converged"]) for r in rows)
+    value = accepted / wall
+
+    avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner)
+
+    if args.states_csv:
+        import csv
+
+        keys = [k for k in gpol.SUMMARY_FIELDS if k in rows[0]] + ["total_iters", "n_setups"]
+        with open(args.states_csv, "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=keys, extrasaction="ignore")
+            w.writeheader()
+            w.writerows(rows)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args, dims, rows)
+
+    out = {
+        "metric": "accepted guarded solves/s vs plain FP64 GMG-FGMRES @ true res<=tol; matvec GB/s",
+        "value": value,
+        "unit": "accepted guarded solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Bernoulli density states)",
+        "config": {
+            "workload": (f"plain FP64 guarded floor policy, cantilever {dims[0]}x{dims[1]}x{dims[2]} "
+                         f"({dims[0] * dims[1] * dims[2]} hex8), SIMP p={args.penal:g}, "
+                         f"{args.load} load, Bernoulli p in {{{args.probabilities}}}, ladder "
+                         "{1e-12,1e-3,1e-2}, restart 300, tol 1e-6"),
+            "grid": list(dims), "n_free": runner.mf_op._n_free,
+            "states": [[r["seed"], r["solid_probability"]] for r in rows],
+            "accepted": accepted,
+            "mean_fgmres_iters_per_state": float(np.mean([r["total_iters"] for r in rows])),
+            "triggers": [r["trigger"] for r in rows],
+            "selected_floors": [r["recommended_rho_min"] for r in rows],
+            "max_true_rel_residual": float(max(r["solve_final_rel_residual"] for r in rows)),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_fine_matvec<double, EPI_PLAIN> (gf_matvec_f64)",
+            "bound": "hbm",
+            "achieved": gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": avg_us,
+            "algorithmic_bytes": bytes_alg,
+            "flops": flops,
+            "fp64_tflops": tfs,
+            "fp64_frac": tfs / FP64_PEAK_TFLOPS,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
