@@ -233,6 +233,8 @@ def main():
             "flops": flops,
             "fp64_tflops": tfs,
             "fp64_frac": tfs / FP64_PEAK_TFLOPS,
+            "launch_shape": runner.mf_op.launch_shape,
+            "autotune_us": runner.mf_op.launch_timings_us,
         },
         "cpu_baseline": cpu,
     }

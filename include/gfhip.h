@@ -71,6 +71,10 @@ typedef struct gf_gmg_params {
   int32_t root_local_correction_inner_restart;
   int32_t inner_krylov_steps;
   int32_t inner_krylov_restart;
+  /* coarsest dense solve (not a reference option): 0 = explicit inverse
+   * L^-T L^-1 formed once per set-up and applied as one parallel GEMV
+   * (default), 1 = the two triangular solves of the reference */
+  int32_t coarsest_solve;
 } gf_gmg_params;
 
 /* State read back after gf_gmg_setup (properties of GalerkinMatFreeGMG,
@@ -102,6 +106,14 @@ const char* gf_last_error(void);
 int32_t gf_ctx_create(gf_ctx** out, int32_t nelx, int32_t nely, int32_t nelz,
                       const double* KE, const int32_t* free_dofs, int32_t n_free, int32_t ndof);
 int32_t gf_ctx_destroy(gf_ctx* ctx);
+/* Launch-shape autotuner of the fine operator (no reference counterpart):
+ * times the one-lane-per-node and four-waves-per-64-nodes kernels on this
+ * grid and keeps the faster for every later call on ctx.  Both shapes give
+ * bitwise-identical results.  us_out (nullable, host): 2 floats of timings;
+ * shape_out (nullable, host): the chosen shape (1 or 4).  Synchronises. */
+int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* stream);
+/* Force a launch shape: 0 = process default, 1, or 4. */
+int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape);
 /* y_free = K_ff(E) u_free. E: n_elem moduli; u,y: n_free (device). */
 int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream);
 int32_t gf_matvec_f32(gf_ctx* ctx, const float* E, const float* u, float* y, void* stream);

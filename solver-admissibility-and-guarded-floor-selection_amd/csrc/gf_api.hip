@@ -48,6 +48,24 @@ int32_t gf_ctx_create(gf_ctx** out, int32_t nelx, int32_t nely, int32_t nelz, co
   return 0;
 }
 
+int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* stream) {
+  CHECK_PTR(ctx);
+  int rc = ctx->c.autotune(S(stream), us_out);
+  if (rc) return rc;
+  if (shape_out) *shape_out = ctx->c.g.shape;
+  return 0;
+}
+
+int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape) {
+  CHECK_PTR(ctx);
+  if (shape != 0 && shape != 1 && shape != 4) {
+    set_error("gf_ctx_set_shape: shape must be 0, 1 or 4");
+    return -2;
+  }
+  ctx->c.g.shape = shape;
+  return 0;
+}
+
 int32_t gf_ctx_destroy(gf_ctx* ctx) {
   if (!ctx) return 0;
   ctx->c.release();
@@ -198,6 +216,7 @@ void gf_gmg_default_params(gf_gmg_params* p) {
   p->root_local_correction_inner_restart = 0;
   p->inner_krylov_steps = 0;
   p->inner_krylov_restart = 0;
+  p->coarsest_solve = 0;
 }
 
 int32_t gf_gmg_create(gf_gmg** out, gf_ctx* ctx, const gf_gmg_params* params) {

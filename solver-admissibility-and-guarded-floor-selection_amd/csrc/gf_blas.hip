@@ -48,34 +48,41 @@ __device__ __forceinline__ void finish_reduce(const double (&v)[NP], double* par
   double bs[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) bs[p] = block_sum(v[p], lds);
+  // partials and ticket live in uncached device memory: a store completes at
+  // the memory, so a work-group release (wait for the store) orders it before
+  // the ticket increment without writing back the whole L2.
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int p = 0; p < NP; ++p) partials[p * kRedBlocks + blockIdx.x] = bs[p];
-    __threadfence();
-    int t = atomicAdd(ticket, 1);
+    for (int p = 0; p < NP; ++p)
+      __hip_atomic_store(&partials[p * kRedBlocks + blockIdx.x], bs[p], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    const int t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (t == (int)gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();
   const int nb = gridDim.x;
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
     double a = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x)
-      a += __builtin_nontemporal_load(&partials[p * kRedBlocks + i]);
+      a += __hip_atomic_load(&partials[p * kRedBlocks + i], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
     double s = block_sum(a, lds);
     if (threadIdx.x == 0) {
       if (post == 1) s = sqrt(s);
       out[p] = s;
     }
   }
-  if (threadIdx.x == 0) *ticket = 0;
+  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Reduction grid: ~4096 elements per block, between 16 and kRedBlocks blocks
+// (measured on MI355X: fewer, fuller blocks shorten the ticket tail).
 inline int red_grid(long n) {
-  int g = cdiv(n, kBlock * 4);
-  return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
+  int g = cdiv(n, 4096);
+  return g < 16 ? 16 : (g > kRedBlocks ? kRedBlocks : g);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,23 +235,25 @@ __global__ __launch_bounds__(kBlock) void k_minmax(long n, const double* __restr
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = smax[0];
-    partials[kRedBlocks + blockIdx.x] = smin[0];
-    __threadfence();
-    last = (atomicAdd(ticket, 1) == (int)gridDim.x - 1);
+    __hip_atomic_store(&partials[blockIdx.x], smax[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&partials[kRedBlocks + blockIdx.x], smin[0], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    last = (__hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            (int)gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();
   if (threadIdx.x == 0) {
     double a = -INFINITY, b = INFINITY;
     for (int i = 0; i < (int)gridDim.x; ++i) {
-      a = fmax(a, __builtin_nontemporal_load(&partials[i]));
-      b = fmin(b, __builtin_nontemporal_load(&partials[kRedBlocks + i]));
+      a = fmax(a, __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      b = fmin(b, __hip_atomic_load(&partials[kRedBlocks + i], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT));
     }
     out2[0] = a;
     out2[1] = b;
-    *ticket = 0;
+    __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -360,17 +369,32 @@ int count_gt(Work& w, long n, const double* E, const double* thr_src, double sca
   return 0;
 }
 
+// Reduction scratch is uncached device memory, recycled through a process-
+// wide free list and never returned to the driver (uncached allocations that
+// kernels have read are kept until the process ends).
+namespace {
+struct UncachedPool {
+  std::vector<std::pair<double*, int*>> free_list;
+} g_pool;
+}  // namespace
+
 int Work::init() {
   if (partials) return 0;
-  GF_HIP(hipMalloc(&partials, sizeof(double) * 4 * kRedBlocks));
-  GF_HIP(hipMalloc(&ticket, sizeof(int) * 4));
-  GF_HIP(hipMemset(ticket, 0, sizeof(int) * 4));
+  if (!g_pool.free_list.empty()) {
+    partials = g_pool.free_list.back().first;
+    ticket = g_pool.free_list.back().second;
+    g_pool.free_list.pop_back();
+    return 0;
+  }
+  GF_HIP(hipExtMallocWithFlags((void**)&partials, sizeof(double) * 4 * kRedBlocks,
+                               hipDeviceMallocUncached));
+  GF_HIP(hipExtMallocWithFlags((void**)&ticket, sizeof(int) * 64, hipDeviceMallocUncached));
+  GF_HIP(hipMemset(ticket, 0, sizeof(int) * 64));
   return 0;
 }
 
 void Work::release() {
-  if (partials) (void)hipFree(partials);
-  if (ticket) (void)hipFree(ticket);
+  if (partials) g_pool.free_list.emplace_back(partials, ticket);
   partials = nullptr;
   ticket = nullptr;
 }

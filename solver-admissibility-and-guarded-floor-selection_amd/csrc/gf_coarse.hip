@@ -126,43 +126,55 @@ __device__ __forceinline__ void coarse_epi(int i, T ax, const T* __restrict__ x,
   }
 }
 
+// One thread per (node, stencil slot): a block covers 8 nodes x 32 lanes
+// (27 slots used); slot products Sum_q A[k][r][q] x[J_k, q] meet in LDS and
+// lane r < 3 of each node adds them in slot order before the epilogue.  The
+// small coarse grids get 27 times the parallelism of a thread-per-node map.
 template <typename T, int EPI>
 __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T* __restrict__ A,
                                                     const T* __restrict__ x, T* __restrict__ y,
                                                     EpiArgs<T> ep, const int* gate) {
   if (gate && *gate == 0) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
-  if ((o0 & o1 & o2) < 0) return;
-  int ix, iy, iz;
-  node_xyz(g, n, ix, iy, iz);
-  const long nn = g.n_nodes;
-  T acc0 = 0, acc1 = 0, acc2 = 0;
-#pragma unroll
-  for (int k = 0; k < 27; ++k) {
+  __shared__ T part[8][27][3];
+  const int k = threadIdx.x & 31, ln = threadIdx.x >> 5;
+  const int n = blockIdx.x * 8 + ln;
+  const bool valid = n < g.n_nodes;
+  if (valid && k < 27) {
+    int ix, iy, iz;
+    node_xyz(g, n, ix, iy, iz);
     const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
-    if (jx < 0 || jx >= g.NX || jy < 0 || jy >= g.NY || jz < 0 || jz >= g.NZ) continue;
-    const int m = (jx * g.NY + jy) * g.NZ + jz;
+    T s0 = 0, s1 = 0, s2 = 0;
+    if (jx >= 0 && jx < g.NX && jy >= 0 && jy < g.NY && jz >= 0 && jz < g.NZ) {
+      const int m = (jx * g.NY + jy) * g.NZ + jz;
+      const long nn = g.n_nodes;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int f = map(3 * m + q);
-      if (f < 0) continue;
-      const T xv = x[f];
-      acc0 += A[(k * 9 + 0 * 3 + q) * nn + n] * xv;
-      acc1 += A[(k * 9 + 1 * 3 + q) * nn + n] * xv;
-      acc2 += A[(k * 9 + 2 * 3 + q) * nn + n] * xv;
+      for (int q = 0; q < 3; ++q) {
+        const int f = map(3 * m + q);
+        if (f < 0) continue;
+        const T xv = x[f];
+        s0 += A[(k * 9 + 0 * 3 + q) * nn + n] * xv;
+        s1 += A[(k * 9 + 1 * 3 + q) * nn + n] * xv;
+        s2 += A[(k * 9 + 2 * 3 + q) * nn + n] * xv;
+      }
     }
+    part[ln][k][0] = s0;
+    part[ln][k][1] = s1;
+    part[ln][k][2] = s2;
   }
-  if (o0 >= 0) coarse_epi<T, EPI>(o0, acc0, x, y, ep);
-  if (o1 >= 0) coarse_epi<T, EPI>(o1, acc1, x, y, ep);
-  if (o2 >= 0) coarse_epi<T, EPI>(o2, acc2, x, y, ep);
+  __syncthreads();
+  if (!valid || k >= 3) return;
+  const int o = map(3 * n + k);
+  if (o < 0) return;
+  T acc = 0;
+#pragma unroll
+  for (int kk = 0; kk < 27; ++kk) acc += part[ln][kk][k];
+  coarse_epi<T, EPI>(o, acc, x, y, ep);
 }
 
 template <typename T>
 int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
                    const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
-  dim3 grid(cdiv(g.n_nodes, kBlock)), block(kBlock);
+  dim3 grid(cdiv(g.n_nodes, 8)), block(kBlock);
   switch (epi) {
     case EPI_PLAIN:
       hipLaunchKernelGGL((k_stencil<T, EPI_PLAIN>), grid, block, 0, st, g, map, A, x, y, ep, gate);
@@ -289,45 +301,56 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid g
   if (o2 >= 0) yf[o2] = acc[2];
 }
 
+// Restriction R = P^T: coarse node I gathers its (up to 27) fine children in
+// ascending fine-node order.  64 coarse nodes x 4 child groups per block,
+// partial sums combined in LDS in group order.
 __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
                                                      const double* __restrict__ rf,
                                                      double* __restrict__ yc, const int* gate) {
   if (gate && *gate == 0) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= gc.n_nodes) return;
-  const int o0 = mapc(3 * n), o1 = mapc(3 * n + 1), o2 = mapc(3 * n + 2);
-  if ((o0 & o1 & o2) < 0) return;
-  int ix, iy, iz;
-  node_xyz(gc, n, ix, iy, iz);
+  __shared__ double part[3][3][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + lane;
+  const bool valid = n < gc.n_nodes;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int a = -1; a <= 1; ++a) {
-    const int fx = 2 * ix + a;
-    if (fx < 0 || fx >= gf.NX) continue;
-    const double wxv = pweight(fx, ix, gc.NX);
-    if (wxv == 0.0) continue;
-    for (int b = -1; b <= 1; ++b) {
-      const int fy = 2 * iy + b;
-      if (fy < 0 || fy >= gf.NY) continue;
-      const double wyv = pweight(fy, iy, gc.NY);
-      if (wyv == 0.0) continue;
-      for (int c = -1; c <= 1; ++c) {
-        const int fz = 2 * iz + c;
-        if (fz < 0 || fz >= gf.NZ) continue;
-        const double wzv = pweight(fz, iz, gc.NZ);
-        if (wzv == 0.0) continue;
-        const double w = (wxv * wyv) * wzv;
-        const int m = (fx * gf.NY + fy) * gf.NZ + fz;
+  int o[3] = {-1, -1, -1};
+  if (valid) {
+    o[0] = mapc(3 * n);
+    o[1] = mapc(3 * n + 1);
+    o[2] = mapc(3 * n + 2);
+    int ix, iy, iz;
+    node_xyz(gc, n, ix, iy, iz);
+    double wx[3], wy[3], wz[3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const int f = mapf(3 * m + q);
-          if (f >= 0) acc[q] += w * rf[f];
-        }
+    for (int a = 0; a < 3; ++a) {
+      const int fx = 2 * ix + a - 1, fy = 2 * iy + a - 1, fz = 2 * iz + a - 1;
+      wx[a] = (fx >= 0 && fx < gf.NX) ? pweight(fx, ix, gc.NX) : 0.0;
+      wy[a] = (fy >= 0 && fy < gf.NY) ? pweight(fy, iy, gc.NY) : 0.0;
+      wz[a] = (fz >= 0 && fz < gf.NZ) ? pweight(fz, iz, gc.NZ) : 0.0;
+    }
+    for (int c27 = grp; c27 < 27; c27 += 4) {
+      const int a = c27 / 9, b = (c27 / 3) % 3, c = c27 % 3;
+      if (wx[a] == 0.0 || wy[b] == 0.0 || wz[c] == 0.0) continue;
+      const double w = (wx[a] * wy[b]) * wz[c];
+      const int m = ((2 * ix + a - 1) * gf.NY + (2 * iy + b - 1)) * gf.NZ + (2 * iz + c - 1);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int f = mapf(3 * m + q);
+        if (f >= 0) acc[q] += w * rf[f];
       }
     }
   }
-  if (o0 >= 0) yc[o0] = acc[0];
-  if (o1 >= 0) yc[o1] = acc[1];
-  if (o2 >= 0) yc[o2] = acc[2];
+  if (grp > 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) part[grp - 1][q][lane] = acc[q];
+  }
+  __syncthreads();
+  if (grp != 0 || !valid) return;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double v = ((acc[q] + part[0][q][lane]) + part[1][q][lane]) + part[2][q][lane];
+    if (o[q] >= 0) yc[o[q]] = v;
+  }
 }
 
 int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
@@ -340,7 +363,7 @@ int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& ma
 
 int restrict_(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
               const double* rf, double* yc, const int* gate, hipStream_t st) {
-  hipLaunchKernelGGL(k_restrict, dim3(cdiv(gc.n_nodes, kBlock)), dim3(kBlock), 0, st, gf, mapf,
+  hipLaunchKernelGGL(k_restrict, dim3(cdiv(gc.n_nodes, 64)), dim3(kBlock), 0, st, gf, mapf,
                      gc, mapc, rf, yc, gate);
   GF_CHECK_LAUNCH();
   return 0;
@@ -499,6 +522,10 @@ int stencil_to_dense(const Grid& g, const DofMap& map, const double* A, int n_fr
   return 0;
 }
 
+// Right-looking Cholesky in one work-group (n <= 5000): for each pivot k the
+// column below it is scaled, then the trailing lower triangle is updated with
+// one thread per matrix entry (consecutive threads on consecutive columns, so
+// the row-major updates coalesce).
 __global__ __launch_bounds__(1024) void k_cholesky(int n, double* __restrict__ D,
                                                    double* __restrict__ Lt, int* fail) {
   __shared__ double piv;
@@ -517,19 +544,23 @@ __global__ __launch_bounds__(1024) void k_cholesky(int n, double* __restrict__ D
     const double p = piv;
     for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) D[(long)i * n + k] /= p;
     __syncthreads();
-    // trailing update of the lower triangle, rows distributed over threads
-    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
-      const double lik = D[(long)i * n + k];
-      double* row = D + (long)i * n;
-      const double* rowk = D;  // column k of rows j read below
-      for (int j = k + 1; j <= i; ++j) row[j] -= lik * rowk[(long)j * n + k];
+    const int m = n - k - 1;  // trailing size
+    // entries (i, j), k < j <= i < n, enumerated row by row
+    const long total = (long)m * (m + 1) / 2;
+    for (long e = threadIdx.x; e < total; e += blockDim.x) {
+      // row r (0-based in the trailing block) with r(r+1)/2 <= e < (r+1)(r+2)/2
+      long r = (long)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+      while ((r + 1) * (r + 2) / 2 <= e) ++r;
+      while (r * (r + 1) / 2 > e) --r;
+      const long c = e - r * (r + 1) / 2;
+      const int i = k + 1 + (int)r, j = k + 1 + (int)c;
+      D[(long)i * n + j] -= D[(long)i * n + k] * D[(long)j * n + k];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) *fail = bad;
   __syncthreads();
   if (bad) return;
-  // zero the strict upper triangle, write the transpose (column-major copy)
   for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
     const int i = (int)(idx / n), j = (int)(idx % n);
     if (j > i) D[idx] = 0.0;
@@ -558,47 +589,183 @@ int dense_add_diag(int n, double* D, double eps, hipStream_t st) {
   return 0;
 }
 
-// x = L^-T L^-1 b, column sweeps; the working vector lives in LDS.
-constexpr int kTrsvThreads = 512;
+// x = L^-T L^-1 b by blocked substitution in one work-group.  The working
+// vector lives in LDS; 64x64 diagonal blocks are staged in LDS and solved by
+// one wavefront (lane r owns row r, column sweep with the stored reciprocal
+// pivots); the off-diagonal updates are 64-term row dot products streamed by
+// all threads (row-major L for the forward sweep, its transpose Lt for the
+// backward sweep, so every row segment is contiguous).
+constexpr int kTrsvThreads = 256;
 constexpr int kTrsvMaxN = 5000;
+constexpr int kTB = 64;
 
 __global__ __launch_bounds__(kTrsvThreads) void k_chol_solve(int n, const double* __restrict__ L,
                                                              const double* __restrict__ Lt,
+                                                             const double* __restrict__ rdiag,
                                                              const double* __restrict__ b,
                                                              double* __restrict__ x,
                                                              const int* gate) {
   if (gate && *gate == 0) return;
   __shared__ double v[kTrsvMaxN];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = b[i];
+  __shared__ double D[kTB * (kTB + 1)];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int i = tid; i < n; i += blockDim.x) v[i] = b[i];
   __syncthreads();
-  // forward: L y = b   (column j of L is row j of Lt: contiguous)
-  for (int j = 0; j < n; ++j) {
-    const double yj = v[j] / Lt[(long)j * n + j];
+  // forward: L y = b
+  for (int kb = 0; kb < n; kb += kTB) {
+    const int m = min(kTB, n - kb);
+    for (int idx = tid; idx < m * m; idx += blockDim.x) {
+      const int r = idx / m, c = idx % m;
+      D[r * (kTB + 1) + c] = (c <= r) ? L[(long)(kb + r) * n + kb + c] : 0.0;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) v[j] = yj;
-    const double* col = Lt + (long)j * n;
-    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) v[i] -= col[i] * yj;
+    if (wid == 0) {
+      double val = lane < m ? v[kb + lane] : 0.0;
+      const double rd = lane < m ? rdiag[kb + lane] : 0.0;
+      for (int c = 0; c < m; ++c) {
+        const double yc = __shfl(val, c, 64) * __shfl(rd, c, 64);
+        if (lane == c) val = yc;
+        else if (lane > c) val -= D[lane * (kTB + 1) + c] * yc;
+      }
+      if (lane < m) v[kb + lane] = val;
+    }
+    __syncthreads();
+    for (int i = kb + m + tid; i < n; i += blockDim.x) {
+      const double* row = L + (long)i * n + kb;
+      double acc = 0.0;
+      for (int c = 0; c < m; ++c) acc += row[c] * v[kb + c];
+      v[i] -= acc;
+    }
     __syncthreads();
   }
-  // backward: L^T x = y  (column j of L^T is row j of L: contiguous)
-  for (int j = n - 1; j >= 0; --j) {
-    const double xj = v[j] / L[(long)j * n + j];
+  // backward: L^T x = y  (block rows from the bottom)
+  const int nb = (n + kTB - 1) / kTB;
+  for (int bi = nb - 1; bi >= 0; --bi) {
+    const int kb = bi * kTB;
+    const int m = min(kTB, n - kb);
+    // U = (L block)^T: U[r][c] = L[kb+c][kb+r] = Lt[(kb+r)*n + kb+c], c >= r
+    for (int idx = tid; idx < m * m; idx += blockDim.x) {
+      const int r = idx / m, c = idx % m;
+      D[r * (kTB + 1) + c] = (c >= r) ? Lt[(long)(kb + r) * n + kb + c] : 0.0;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) v[j] = xj;
-    const double* row = L + (long)j * n;
-    for (int i = threadIdx.x; i < j; i += blockDim.x) v[i] -= row[i] * xj;
+    if (wid == 0) {
+      double val = lane < m ? v[kb + lane] : 0.0;
+      const double rd = lane < m ? rdiag[kb + lane] : 0.0;
+      for (int c = m - 1; c >= 0; --c) {
+        const double xc = __shfl(val, c, 64) * __shfl(rd, c, 64);
+        if (lane == c) val = xc;
+        else if (lane < c) val -= D[lane * (kTB + 1) + c] * xc;
+      }
+      if (lane < m) v[kb + lane] = val;
+    }
+    __syncthreads();
+    for (int i = tid; i < kb; i += blockDim.x) {
+      const double* row = Lt + (long)i * n + kb;
+      double acc = 0.0;
+      for (int c = 0; c < m; ++c) acc += row[c] * v[kb + c];
+      v[i] -= acc;
+    }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = v[i];
+  for (int i = tid; i < n; i += blockDim.x) x[i] = v[i];
 }
 
-int dense_chol_solve(int n, const double* L, const double* Lt, const double* b, double* x,
-                     const int* gate, hipStream_t st) {
+int dense_chol_solve(int n, const double* L, const double* Lt, const double* rdiag,
+                     const double* b, double* x, const int* gate, hipStream_t st) {
   if (n > kTrsvMaxN) {
     set_error("dense_chol_solve: n exceeds the dense coarsest limit");
     return -2;
   }
-  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(kTrsvThreads), 0, st, n, L, Lt, b, x, gate);
+  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(kTrsvThreads), 0, st, n, L, Lt, rdiag, b, x,
+                     gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+__global__ void k_recip_diag(int n, const double* __restrict__ L, double* __restrict__ r) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    r[i] = 1.0 / L[(long)i * n + i];
+}
+
+// Y = L^{-1}: column j of Y by forward substitution of e_j (one thread per
+// column; Y stored row-major, Y[i][j] for i >= j).
+__global__ __launch_bounds__(kBlock) void k_tri_inverse(int n, const double* __restrict__ L,
+                                                        double* __restrict__ Y) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  for (int i = 0; i < j; ++i) Y[(long)i * n + j] = 0.0;
+  for (int i = j; i < n; ++i) {
+    double acc = (i == j) ? 1.0 : 0.0;
+    const double* row = L + (long)i * n;
+    for (int k = j; k < i; ++k) acc -= row[k] * Y[(long)k * n + j];
+    Y[(long)i * n + j] = acc / row[i];
+  }
+}
+
+// Kinv = Y^T Y (symmetric), one thread per entry; Yt = Y^T row-major makes
+// both operands of every dot product contiguous.
+__global__ __launch_bounds__(kBlock) void k_yty(int n, const double* __restrict__ Yt,
+                                                double* __restrict__ K) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const int k0 = i > j ? i : j;  // Y[k][i] = 0 for k < i
+  const double* a = Yt + (long)i * n;
+  const double* b = Yt + (long)j * n;
+  double acc = 0.0;
+  for (int k = k0; k < n; ++k) acc += a[k] * b[k];
+  K[e] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_transpose(int n, const double* __restrict__ A,
+                                                      double* __restrict__ B) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  B[(long)j * n + i] = A[e];
+}
+
+// Kinv = L^{-T} L^{-1} from the lower Cholesky factor L (row-major).  Y and
+// Yt are n*n scratch buffers.
+int dense_inverse_from_chol(int n, const double* L, double* Y, double* Yt, double* Kinv,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_tri_inverse, dim3(cdiv(n, 64)), dim3(64), 0, st, n, L, Y);
+  GF_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_transpose, dim3(cdiv((long)n * n, kBlock)), dim3(kBlock), 0, st, n, Y, Yt);
+  GF_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_yty, dim3(cdiv((long)n * n, kBlock)), dim3(kBlock), 0, st, n, Yt, Kinv);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+// x = Kinv b: one wavefront per row, lanes stride the row, fixed-order
+// wave reduction.
+__global__ __launch_bounds__(kBlock) void k_dense_gemv(int n, const double* __restrict__ K,
+                                                       const double* __restrict__ b,
+                                                       double* __restrict__ x, const int* gate) {
+  if (gate && *gate == 0) return;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double* r = K + (long)row * n;
+  double acc = 0.0;
+  for (int c = lane; c < n; c += 64) acc += r[c] * b[c];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if (lane == 0) x[row] = acc;
+}
+
+int dense_gemv(int n, const double* K, const double* b, double* x, const int* gate,
+               hipStream_t st) {
+  hipLaunchKernelGGL(k_dense_gemv, dim3(cdiv(n, kBlock / 64)), dim3(kBlock), 0, st, n, K, b, x,
+                     gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int dense_recip_diag(int n, const double* L, double* rdiag, hipStream_t st) {
+  hipLaunchKernelGGL(k_recip_diag, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, n, L, rdiag);
   GF_CHECK_LAUNCH();
   return 0;
 }

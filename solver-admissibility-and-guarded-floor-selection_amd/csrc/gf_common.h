@@ -54,6 +54,7 @@ struct Grid {
   int NX, NY, NZ;      // nodes = elements + 1
   int n_nodes;
   int ndof;
+  int shape;           // fine-operator launch shape (0 = process default)
 };
 
 inline Grid make_grid(int nx, int ny, int nz) {
@@ -62,6 +63,7 @@ inline Grid make_grid(int nx, int ny, int nz) {
   g.NX = nx + 1; g.NY = ny + 1; g.NZ = nz + 1;
   g.n_nodes = g.NX * g.NY * g.NZ;
   g.ndof = 3 * g.n_nodes;
+  g.shape = 0;
   return g;
 }
 

@@ -57,24 +57,48 @@ __device__ __forceinline__ void apply_epilogue(int i, T ax, const T* __restrict_
   }
 }
 
+// Element contributions are summed with one fixed tree for every launch
+// shape:  P_ab = E_ab0*t_ab0 + E_ab1*t_ab1  (element (a,b,c) = (ix-1+a,
+// iy-1+b, iz-1+c)),  Q_a = P_a0 + P_a1,  y = Q_0 + Q_1,  each t the 24-term
+// row product in column order.  SPLIT lanes cooperate on one node (SPLIT=1:
+// a whole node per lane; 2: one lane per a; 4: one lane per (a,b)) and
+// combine their partial trees with shuffles, so all shapes are bitwise
+// identical and the shape can be autotuned freely.
+template <typename T>
+__device__ __forceinline__ void elem_rows(const T* __restrict__ Ke, const T (&un)[27][3], int a,
+                                          int b, int c, T& t0, T& t1, T& t2) {
+  const int L = lidx(1 - a, 1 - b, 1 - c);
+  t0 = 0;
+  t1 = 0;
+  t2 = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = (a + lpos(j, 0)) * 9 + (b + lpos(j, 1)) * 3 + (c + lpos(j, 2));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T uu = un[k][q];
+      t0 += Ke[(3 * L + 0) * 24 + 3 * j + q] * uu;
+      t1 += Ke[(3 * L + 1) * 24 + 3 * j + q] * uu;
+      t2 += Ke[(3 * L + 2) * 24 + 3 * j + q] * uu;
+    }
+  }
+}
+
+// SPLIT = 1: one lane per node, the whole tree in registers.
 template <typename T, int EPI>
-__global__ __launch_bounds__(kBlock) void k_fine_matvec(Grid g, DofMap map,
-                                                        const T* __restrict__ Ke,
-                                                        const T* __restrict__ E,
-                                                        const T* __restrict__ u,
-                                                        T* __restrict__ y, EpiArgs<T> ep,
-                                                        const int* gate) {
+__global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
+                                                             const T* __restrict__ Ke,
+                                                             const T* __restrict__ E,
+                                                             const T* __restrict__ u,
+                                                             T* __restrict__ y, EpiArgs<T> ep,
+                                                             const int* gate) {
   using A = typename Acc<T>::type;
   if (gate && *gate == 0) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.n_nodes) return;
-  const int iz = n % g.NZ;
-  const int t = n / g.NZ;
-  const int iy = t % g.NY;
-  const int ix = t / g.NY;
   const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
-  if ((o0 & o1 & o2) < 0) return;  // all three components fixed
-
+  if ((o0 & o1 & o2) < 0) return;
+  const int iz = n % g.NZ, t = n / g.NZ, iy = t % g.NY, ix = t / g.NY;
   T un[27][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
@@ -86,81 +110,164 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec(Grid g, DofMap map,
         const bool in = (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
                         (jz < g.NZ);
         const int m = (jx * g.NY + jy) * g.NZ + jz;
-        const int k = dx * 9 + dy * 3 + dz;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const int f = in ? map(3 * m + c) : -1;
-          un[k][c] = f >= 0 ? u[f] : T(0);
+          un[dx * 9 + dy * 3 + dz][c] = f >= 0 ? u[f] : T(0);
         }
       }
-  T Ee[8];
+  A Q[2][3];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 2; ++a) {
+    A P[2][3];
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 2; ++b) {
+      P[b][0] = 0;
+      P[b][1] = 0;
+      P[b][2] = 0;
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const int ex = ix - 1 + a, ey = iy - 1 + b, ez = iz - 1 + c;
         const bool in = (ex >= 0) & (ex < g.nx) & (ey >= 0) & (ey < g.ny) & (ez >= 0) &
                         (ez < g.nz);
-        Ee[a * 4 + b * 2 + c] = in ? E[(ex * g.ny + ey) * g.nz + ez] : T(0);
+        const T e = in ? E[(ex * g.ny + ey) * g.nz + ez] : T(0);
+        T t0, t1, t2;
+        elem_rows<T>(Ke, un, a, b, c, t0, t1, t2);
+        P[b][0] += (A)(e * t0);
+        P[b][1] += (A)(e * t1);
+        P[b][2] += (A)(e * t2);
       }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) Q[a][r] = P[0][r] + P[1][r];
+  }
+  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)(Q[0][0] + Q[1][0]), u, y, ep);
+  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)(Q[0][1] + Q[1][1]), u, y, ep);
+  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)(Q[0][2] + Q[1][2]), u, y, ep);
+}
 
-  A acc0 = 0, acc1 = 0, acc2 = 0;
+// SPLIT = 4: a block is 64 nodes x 4 waves; wave w computes the element pair
+// (a,b) = (w>>1, w&1) (c = 0,1) for its node, so the element-local node index,
+// and with it every Ke address, is wave-uniform (scalar loads, SGPR operands),
+// and each lane loads only the 2x2x3 node window of its two elements.  The
+// pair sums P_ab meet in LDS and wave 0 forms (P00+P01)+(P10+P11), the same
+// tree as the one-lane-per-node kernel.
+template <typename T>
+__device__ __forceinline__ void pair_rows(const T* __restrict__ Ke, const T (&w)[2][2][3][3],
+                                          int a, int b, int c, T& t0, T& t1, T& t2) {
+  const int L = (1 - c) * 4 + ((1 - b) ? ((1 - a) ? 2 : 3) : ((1 - a) ? 1 : 0));
+  const T* K0 = Ke + (3 * L + 0) * 24;
+  const T* K1 = Ke + (3 * L + 1) * 24;
+  const T* K2 = Ke + (3 * L + 2) * 24;
+  t0 = 0;
+  t1 = 0;
+  t2 = 0;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int j = 0; j < 8; ++j) {
+    const int px = lpos(j, 0), py = lpos(j, 1), pz = lpos(j, 2);
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int q = 0; q < 3; ++q) {
+      const T uu = w[px][py][c + pz][q];
+      t0 += K0[3 * j + q] * uu;
+      t1 += K1[3 * j + q] * uu;
+      t2 += K2[3 * j + q] * uu;
+    }
+  }
+}
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map,
+                                                              const T* __restrict__ Ke,
+                                                              const T* __restrict__ E,
+                                                              const T* __restrict__ u,
+                                                              T* __restrict__ y, EpiArgs<T> ep,
+                                                              const int* gate) {
+  using A = typename Acc<T>::type;
+  if (gate && *gate == 0) return;
+  __shared__ A part[3][3][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int a = wv >> 1, b = wv & 1;
+  const int n = blockIdx.x * 64 + lane;
+  const bool valid = n < g.n_nodes;
+  const int nn = valid ? n : 0;
+  const int iz = nn % g.NZ, t = nn / g.NZ, iy = t % g.NY, ix = t / g.NY;
+  // node window: x in {ix-1+a, ix+a}, y in {iy-1+b, iy+b}, z in {iz-1, iz, iz+1}
+  T w[2][2][3][3];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int L = lidx(1 - a, 1 - b, 1 - c);
-        T t0 = 0, t1 = 0, t2 = 0;
+  for (int px = 0; px < 2; ++px)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int k = (a + lpos(j, 0)) * 9 + (b + lpos(j, 1)) * 3 + (c + lpos(j, 2));
+    for (int py = 0; py < 2; ++py)
 #pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            const T uu = un[k][q];
-            t0 += Ke[(3 * L + 0) * 24 + 3 * j + q] * uu;
-            t1 += Ke[(3 * L + 1) * 24 + 3 * j + q] * uu;
-            t2 += Ke[(3 * L + 2) * 24 + 3 * j + q] * uu;
-          }
+      for (int pz = 0; pz < 3; ++pz) {
+        const int jx = ix - 1 + a + px, jy = iy - 1 + b + py, jz = iz - 1 + pz;
+        const bool in = valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
+                        (jz < g.NZ);
+        const int m = (jx * g.NY + jy) * g.NZ + jz;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int f = in ? map(3 * m + q) : -1;
+          w[px][py][pz][q] = f >= 0 ? u[f] : T(0);
         }
-        const T e = Ee[a * 4 + b * 2 + c];
-        acc0 += (A)(e * t0);
-        acc1 += (A)(e * t1);
-        acc2 += (A)(e * t2);
       }
-  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)acc0, u, y, ep);
-  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)acc1, u, y, ep);
-  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)acc2, u, y, ep);
+  A P[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int ex = ix - 1 + a, ey = iy - 1 + b, ez = iz - 1 + c;
+    const bool in = valid & (ex >= 0) & (ex < g.nx) & (ey >= 0) & (ey < g.ny) & (ez >= 0) &
+                    (ez < g.nz);
+    const T e = in ? E[(ex * g.ny + ey) * g.nz + ez] : T(0);
+    T t0, t1, t2;
+    pair_rows<T>(Ke, w, a, b, c, t0, t1, t2);
+    P[0] += (A)(e * t0);
+    P[1] += (A)(e * t1);
+    P[2] += (A)(e * t2);
+  }
+  if (wv > 0) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) part[wv - 1][r][lane] = P[r];
+  }
+  __syncthreads();
+  if (wv != 0 || !valid) return;
+  const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
+  A r[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    r[q] = (P[q] + part[0][q][lane]) + (part[1][q][lane] + part[2][q][lane]);
+  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)r[0], u, y, ep);
+  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)r[1], u, y, ep);
+  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
+}
+
+static int g_split_f64 = 4, g_split_f32 = 4;
+
+void set_fine_split(int split_f64, int split_f32) {
+  if (split_f64 == 1 || split_f64 == 4) g_split_f64 = split_f64;
+  if (split_f32 == 1 || split_f32 == 4) g_split_f32 = split_f32;
+}
+int get_fine_split(int is_f64) { return is_f64 ? g_split_f64 : g_split_f32; }
+
+template <typename T, int EPI>
+static void launch_fine(int split, const Grid& g, const DofMap& map, const T* Ke, const T* E,
+                        const T* u, T* y, const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
+  if (split == 1)
+    hipLaunchKernelGGL((k_fine_matvec_node<T, EPI>), dim3(cdiv(g.n_nodes, kBlock)), dim3(kBlock),
+                       0, st, g, map, Ke, E, u, y, ep, gate);
+  else
+    hipLaunchKernelGGL((k_fine_matvec_wave4<T, EPI>), dim3(cdiv(g.n_nodes, 64)), dim3(kBlock), 0,
+                       st, g, map, Ke, E, u, y, ep, gate);
 }
 
 template <typename T>
 int fine_matvec(const Grid& g, const DofMap& map, const T* Ke, const T* E, const T* u, T* y,
                 int epi, const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
-  dim3 grid(cdiv(g.n_nodes, kBlock)), block(kBlock);
+  const int split = g.shape ? g.shape : get_fine_split(sizeof(T) == 8);
   switch (epi) {
-    case EPI_PLAIN:
-      hipLaunchKernelGGL((k_fine_matvec<T, EPI_PLAIN>), grid, block, 0, st, g, map, Ke, E, u, y,
-                         ep, gate);
-      break;
-    case EPI_RESID:
-      hipLaunchKernelGGL((k_fine_matvec<T, EPI_RESID>), grid, block, 0, st, g, map, Ke, E, u, y,
-                         ep, gate);
-      break;
-    case EPI_CHEB0:
-      hipLaunchKernelGGL((k_fine_matvec<T, EPI_CHEB0>), grid, block, 0, st, g, map, Ke, E, u, y,
-                         ep, gate);
-      break;
-    case EPI_CHEB:
-      hipLaunchKernelGGL((k_fine_matvec<T, EPI_CHEB>), grid, block, 0, st, g, map, Ke, E, u, y,
-                         ep, gate);
-      break;
-    case EPI_JACOBI:
-      hipLaunchKernelGGL((k_fine_matvec<T, EPI_JACOBI>), grid, block, 0, st, g, map, Ke, E, u, y,
-                         ep, gate);
-      break;
+    case EPI_PLAIN: launch_fine<T, EPI_PLAIN>(split, g, map, Ke, E, u, y, ep, gate, st); break;
+    case EPI_RESID: launch_fine<T, EPI_RESID>(split, g, map, Ke, E, u, y, ep, gate, st); break;
+    case EPI_CHEB0: launch_fine<T, EPI_CHEB0>(split, g, map, Ke, E, u, y, ep, gate, st); break;
+    case EPI_CHEB: launch_fine<T, EPI_CHEB>(split, g, map, Ke, E, u, y, ep, gate, st); break;
+    case EPI_JACOBI: launch_fine<T, EPI_JACOBI>(split, g, map, Ke, E, u, y, ep, gate, st); break;
     default:
       set_error("fine_matvec: bad epilogue");
       return -2;

@@ -3,6 +3,7 @@
 
 #include "gf_common.h"
 
+#include <utility>
 #include <vector>
 
 namespace gf {
@@ -63,6 +64,11 @@ int count_gt(Work& w, long n, const double* E, const double* thr_src, double sca
 template <typename T>
 int fine_matvec(const Grid& g, const DofMap& map, const T* Ke, const T* E, const T* u, T* y,
                 int epi, const EpiArgs<T>& ep, const int* gate, hipStream_t st);
+// Launch shape of the fine operator kernel: 1 = one lane per node, 4 = four
+// waves per 64 nodes; both give bitwise-identical results.  Process-wide,
+// chosen by the autotuner.
+void set_fine_split(int split_f64, int split_f32);
+int get_fine_split(int is_f64);
 // Zero-start first Chebyshev/Jacobi step: dv = dinv*b; x = a*dv (dv may be null).
 template <typename T>
 int zero_start_step(long n, const T* b, const T* dinv, T a, T* dv, T* x, const int* gate,
@@ -93,8 +99,13 @@ int stencil_to_dense(const Grid& g, const DofMap& map, const double* A, int n_fr
                      hipStream_t st);
 int dense_cholesky(int n, double* D, double* Lt, int* fail, hipStream_t st);
 int dense_add_diag(int n, double* D, double eps, hipStream_t st);
-int dense_chol_solve(int n, const double* L, const double* Lt, const double* b, double* x,
-                     const int* gate, hipStream_t st);
+int dense_chol_solve(int n, const double* L, const double* Lt, const double* rdiag,
+                     const double* b, double* x, const int* gate, hipStream_t st);
+int dense_recip_diag(int n, const double* L, double* rdiag, hipStream_t st);
+int dense_inverse_from_chol(int n, const double* L, double* Y, double* Yt, double* Kinv,
+                            hipStream_t st);
+int dense_gemv(int n, const double* K, const double* b, double* x, const int* gate,
+               hipStream_t st);
 int fixed_pcg(const Grid& g, const DofMap& map, const double* A, const double* dinv,
               int n_free, int iters, const double* b, double* x, double* work,
               const int* gate, hipStream_t st);

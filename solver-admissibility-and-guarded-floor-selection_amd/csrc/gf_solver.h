@@ -30,6 +30,8 @@ struct Ctx {
   int init(int nelx, int nely, int nelz, const double* Ke_host, const int* free_host, int nfree,
            int ndof);
   int nodeblock_alloc();
+  // time the fine-operator launch shapes on this grid, keep the fastest
+  int autotune(hipStream_t st, float* us_out);
   void release();
 };
 
@@ -93,6 +95,9 @@ struct Gmg {
   int chol_regularised = 0;
   double* CH = nullptr;       // dense L (row-major)
   double* CHt = nullptr;      // its transpose
+  double* CHr = nullptr;      // reciprocal pivots 1/L_ii
+  double* CHi = nullptr;      // K^-1 = L^-T L^-1 (coarsest_solve == 0)
+  double* CHy = nullptr;      // scratch for the inverse
   int ch_n = 0;
   int* d_fail = nullptr;
   double* pcg_work = nullptr;

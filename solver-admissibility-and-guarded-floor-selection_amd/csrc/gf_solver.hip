@@ -356,6 +356,50 @@ int Ctx::nodeblock_alloc() {
   return 0;
 }
 
+int Ctx::autotune(hipStream_t st, float* us_out) {
+  const long ne = (long)g.nx * g.ny * g.nz;
+  double *E = nullptr, *u = nullptr, *y = nullptr;
+  GF_TRY(dalloc(&E, ne));
+  GF_TRY(dalloc(&u, n_free));
+  GF_TRY(dalloc(&y, n_free));
+  hipLaunchKernelGGL(k_fill, dim3(256), dim3(kBlock), 0, st, ne, E, 1.0, (const int*)nullptr);
+  hipLaunchKernelGGL(k_fill, dim3(256), dim3(kBlock), 0, st, (long)n_free, u, 1.0,
+                     (const int*)nullptr);
+  hipEvent_t e0, e1;
+  GF_HIP(hipEventCreate(&e0));
+  GF_HIP(hipEventCreate(&e1));
+  const int shapes[2] = {1, 4};
+  float best = 1e30f;
+  int best_shape = 4;
+  for (int s : shapes) {
+    Grid gg = g;
+    gg.shape = s;
+    for (int w = 0; w < 3; ++w)
+      GF_TRY(fine_matvec<double>(gg, map(), Ke, E, u, y, EPI_PLAIN, EpiArgs<double>{}, nullptr, st));
+    GF_HIP(hipEventRecord(e0, st));
+    const int reps = 20;
+    for (int r = 0; r < reps; ++r)
+      GF_TRY(fine_matvec<double>(gg, map(), Ke, E, u, y, EPI_PLAIN, EpiArgs<double>{}, nullptr, st));
+    GF_HIP(hipEventRecord(e1, st));
+    GF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const float us = ms * 1000.f / reps;
+    if (us_out) us_out[s == 1 ? 0 : 1] = us;
+    if (us < best) {
+      best = us;
+      best_shape = s;
+    }
+  }
+  g.shape = best_shape;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(E);
+  dfree(u);
+  dfree(y);
+  return 0;
+}
+
 void Ctx::release() {
   dfree(d_f2f);
   dfree(Ke);
@@ -633,6 +677,9 @@ void Gmg::release() {
   dfree(Q);
   dfree(CH);
   dfree(CHt);
+  dfree(CHr);
+  dfree(CHi);
+  dfree(CHy);
   dfree(d_fail);
   dfree(pcg_work);
   dfree(re_w);
@@ -810,7 +857,10 @@ int Gmg::solve_coarsest(const double* rhs, double* out, const int* gate, hipStre
   double* dst = out ? out : c.X[0];
   if (L == 1) return smooth(0, nullptr, rhs, out, 1, gate, st, result);
   if (chol_mode == 1) {
-    GF_TRY(dense_chol_solve(ch_n, CH, CHt, rhs, dst, gate, st));
+    if (p.coarsest_solve == 0)
+      GF_TRY(dense_gemv(ch_n, CHi, rhs, dst, gate, st));
+    else
+      GF_TRY(dense_chol_solve(ch_n, CH, CHt, CHr, rhs, dst, gate, st));
   } else {
     int it = p.coarse_pcg_iters;
     if (high) it = std::max(p.coarse_pcg_iters, 200);
@@ -1158,8 +1208,16 @@ int Gmg::setup(const double* E, const double* fine_diag, hipStream_t st) {
       if (ch_n != n) {
         dfree(CH);
         dfree(CHt);
+        dfree(CHr);
+        dfree(CHi);
+        dfree(CHy);
         GF_TRY(dalloc(&CH, (size_t)n * n));
         GF_TRY(dalloc(&CHt, (size_t)n * n));
+        GF_TRY(dalloc(&CHr, (size_t)n));
+        if (p.coarsest_solve == 0) {
+          GF_TRY(dalloc(&CHi, (size_t)n * n));
+          GF_TRY(dalloc(&CHy, (size_t)n * n));
+        }
         ch_n = n;
       }
       GF_TRY(stencil_to_dense(c.g, c.map(), c.A, n, CH, st));
@@ -1193,6 +1251,11 @@ int Gmg::setup(const double* E, const double* fine_diag, hipStream_t st) {
       }
     } else {
       chol_mode = 2;
+    }
+    if (chol_mode == 1) {
+      GF_TRY(dense_recip_diag(n, CH, CHr, st));
+      // Y = L^-1 in CHy, its transpose in CHt (the triangular path is unused here)
+      if (p.coarsest_solve == 0) GF_TRY(dense_inverse_from_chol(n, CH, CHy, CHt, CHi, st));
     }
     if (chol_mode == 2 && !pcg_work) GF_TRY(dalloc(&pcg_work, 3L * c.n_free));
   }

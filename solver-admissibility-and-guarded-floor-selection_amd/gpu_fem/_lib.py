@@ -45,6 +45,7 @@ class gf_gmg_params(C.Structure):
         ("root_local_correction_inner_restart", C.c_int32),
         ("inner_krylov_steps", C.c_int32),
         ("inner_krylov_restart", C.c_int32),
+        ("coarsest_solve", C.c_int32),
     ]
 
 
@@ -77,6 +78,8 @@ SIGNATURES = {
     "gf_last_error": [],
     "gf_ctx_create": [C.POINTER(_P), _I, _I, _I, _P, _P, _I, _I],
     "gf_ctx_destroy": [_P],
+    "gf_ctx_autotune": [_P, _P, C.POINTER(_I), _P],
+    "gf_ctx_set_shape": [_P, _I],
     "gf_matvec_f64": [_P, _P, _P, _P, _P],
     "gf_matvec_f32": [_P, _P, _P, _P, _P],
     "gf_residual_f64": [_P, _P, _P, _P, _P, _P],

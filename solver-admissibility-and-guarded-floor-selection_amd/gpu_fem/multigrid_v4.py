@@ -57,7 +57,7 @@ class GalerkinMatFreeGMG:
                  root_local_correction_max_scale: float = 1.0,
                  root_local_correction_inner_steps: int = 10,
                  root_local_correction_inner_restart=None, inner_krylov_steps: int = 0,
-                 inner_krylov_restart=None) -> None:
+                 inner_krylov_restart=None, coarsest_solve: str = "inverse") -> None:
         # argument validation (multigrid_v4.py:600-699), same messages
         if smoother_type not in {"chebyshev", "jacobi"}:
             raise ValueError(f"smoother_type must be 'chebyshev' or 'jacobi'; got {smoother_type!r}")
@@ -101,6 +101,8 @@ class GalerkinMatFreeGMG:
                              f"got {inner_krylov_restart!r}")
         if fine_smoother not in _PREC:
             raise ValueError(f"Unsupported fine smoother '{fine_smoother}'")
+        if coarsest_solve not in ("inverse", "triangular"):
+            raise ValueError("coarsest_solve must be 'inverse' or 'triangular'")
         if tuple(mf_op.grid) != (int(nelx), int(nely), int(nelz)):
             raise ValueError("nelx/nely/nelz do not match the operator's grid")
         if not np.array_equal(_host(KE_UNIT, np.float64).reshape(24, 24), mf_op._ctx.KE):
@@ -154,6 +156,10 @@ class GalerkinMatFreeGMG:
                                                  if root_local_correction_inner_restart else 0)
         p.inner_krylov_steps = int(inner_krylov_steps)
         p.inner_krylov_restart = int(inner_krylov_restart) if inner_krylov_restart else 0
+        # coarsest dense solve: the inverse L^-T L^-1 formed once per set-up and
+        # applied as one GEMV ("inverse"), or the reference's two triangular
+        # solves ("triangular"); both from the same Cholesky factor
+        p.coarsest_solve = 0 if coarsest_solve == "inverse" else 1
         self._params = p
         h = C.c_void_p()
         _lib.check(lib.gf_gmg_create(C.byref(h), mf_op.ctx_handle, C.byref(p)), "gf_gmg_create")

@@ -84,7 +84,8 @@ class FineOperator:
 class MatrixFreeKff:
     """y_free = K_ff(E) u_free without assembling K (solver_v2.py:1121-1251)."""
 
-    def __init__(self, edof_gpu, KE_unit_gpu, free_gpu, n_free: int, ndof: int, *, grid=None):
+    def __init__(self, edof_gpu, KE_unit_gpu, free_gpu, n_free: int, ndof: int, *, grid=None,
+                 autotune: bool = True):
         self._edof_gpu = edof_gpu
         self._KE_unit = KE_unit_gpu
         self._free_gpu = free_gpu
@@ -101,6 +102,21 @@ class MatrixFreeKff:
             raise ValueError("len(free_gpu) != n_free")
         self._ctx = _Ctx(self.grid, KE_unit_gpu, free, ndof)
         self._KE_diag = np.diag(self._ctx.KE).copy()
+        self.launch_shape = None
+        self.launch_timings_us = None
+        if autotune:
+            self.autotune()
+
+    def autotune(self):
+        """Pick the faster fine-operator launch shape for this grid (results
+        are bitwise identical for every shape)."""
+        us = (C.c_float * 2)()
+        shape = C.c_int32()
+        _lib.call("gf_ctx_autotune", self._ctx.h, C.cast(us, C.c_void_p), C.byref(shape),
+                  _lib.stream_handle())
+        self.launch_shape = int(shape.value)
+        self.launch_timings_us = {"node": float(us[0]), "wave4": float(us[1])}
+        return self.launch_shape
 
     @property
     def ctx_handle(self):

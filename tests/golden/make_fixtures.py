@@ -60,6 +60,17 @@ def main(ref_root):
         w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
         w.writeheader()
         w.writerows(rows)
+    # optimized-density input C64_MF (80x40x20 cantilever), data file of the reference
+    import shutil
+    shutil.copyfile(os.path.join(ref_root, "data", "optimized_density_inputs", "C64_MF__rho_final.npy"),
+                    os.path.join(out, "C64_MF__rho_final.npy"))
+    name = "optimized_density_C64_MF_strict_true_residual__density_detector_summary.csv"
+    with open(os.path.join(src, name), newline="") as f:
+        rows = list(csv.DictReader(f))
+    with open(os.path.join(out, "optimized_density_C64_MF.csv"), "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
+        w.writeheader()
+        w.writerows(rows)
     # CPU direct conditioning table (24x12x6 cantilever eigenvalues)
     name = "direct_floor_conditioning__direct_floor_conditioning.csv"
     with open(os.path.join(src, name), newline="") as f:
