@@ -75,6 +75,9 @@ typedef struct gf_gmg_params {
    * L^-T L^-1 formed once per set-up and applied as one parallel GEMV
    * (default), 1 = the two triangular solves of the reference */
   int32_t coarsest_solve;
+  /* 1 = record the whole preconditioner application as a hipGraph after each
+   * set-up and replay it (default); 0 = launch kernel by kernel */
+  int32_t graph_apply;
 } gf_gmg_params;
 
 /* State read back after gf_gmg_setup (properties of GalerkinMatFreeGMG,

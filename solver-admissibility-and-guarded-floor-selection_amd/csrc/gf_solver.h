@@ -117,9 +117,20 @@ struct Gmg {
   double *pw_v = nullptr, *pw_d = nullptr, *pw_kv = nullptr, *pw_av = nullptr, *pw_s = nullptr;
   Work work;
 
+  // hipGraph of one preconditioner application (rhs in gr_in, result in gr_out)
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  double *gr_in = nullptr, *gr_out = nullptr;
+  int graph_builds = 0;
+
   int create(Ctx* c, const gf_gmg_params* params);
   int setup(const double* E, const double* fine_diag, hipStream_t st);
   int apply(const double* rhs, double* out, hipStream_t st);
+  int apply_impl(const double* rhs, double* out, hipStream_t st);
+  int capture();
+  void drop_graph();
   int base_cycle(const double* rhs, double* out, const int* gate, hipStream_t st);
   int outer_solve(const double* b, double* x, int x0_given, double tol, int maxiter, int restart,
                   double* history, int hist_cap, int* n_hist, int* iters, int* converged,

@@ -626,6 +626,8 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
   GF_TRY(dalloc(&ap_corr, n0));
   GF_TRY(dalloc(&ap_trial, n0));
   GF_TRY(dalloc(&ap_st, 16));
+  GF_TRY(dalloc(&gr_in, n0));
+  GF_TRY(dalloc(&gr_out, n0));
   GF_TRY(dalloc(&pw_v, n0));
   GF_TRY(dalloc(&pw_d, n0));
   GF_TRY(dalloc(&pw_kv, n0));
@@ -690,6 +692,11 @@ void Gmg::release() {
   dfree(ap_corr);
   dfree(ap_trial);
   dfree(ap_st);
+  drop_graph();
+  if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  cap_stream = nullptr;
+  dfree(gr_in);
+  dfree(gr_out);
   dfree(pw_v);
   dfree(pw_d);
   dfree(pw_kv);
@@ -1023,11 +1030,51 @@ int Gmg::fgmres_dev(Krylov& K, const std::function<int(const double*, double*, c
   return 0;
 }
 
+void Gmg::drop_graph() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  gexec = nullptr;
+  graph = nullptr;
+  graph_valid = false;
+}
+
+// Record apply_impl(gr_in -> gr_out) on a private stream.  The sequence is
+// static for a given set-up (all decisions are device-side), so one graph is
+// replayed until the next set-up.  It contains kernels only.
+int Gmg::capture() {
+  drop_graph();
+  if (!cap_stream) GF_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+  GF_HIP(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+  const int rc = apply_impl(gr_in, gr_out, cap_stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(cap_stream, &g);
+  if (rc != 0 || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc == 0) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return rc ? rc : -1;
+  }
+  graph = g;
+  GF_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  graph_valid = true;
+  ++graph_builds;
+  return 0;
+}
+
 int Gmg::apply(const double* rhs, double* out, hipStream_t st) {
   if (!is_setup) {
     set_error("GalerkinMatFreeGMG.setup() must be called before apply().");
     return -3;
   }
+  if (!p.graph_apply) return apply_impl(rhs, out, st);
+  if (!graph_valid) GF_TRY(capture());
+  const size_t bytes = sizeof(double) * lv[0].n_free;
+  GF_HIP(hipMemcpyAsync(gr_in, rhs, bytes, hipMemcpyDeviceToDevice, st));
+  GF_HIP(hipGraphLaunch(gexec, st));
+  GF_HIP(hipMemcpyAsync(out, gr_out, bytes, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+int Gmg::apply_impl(const double* rhs, double* out, hipStream_t st) {
   const long n = lv[0].n_free;
   const double* xp;
   if (p.inner_krylov_steps > 0) {
@@ -1136,6 +1183,7 @@ int Gmg::estimate_level_lambda(int l, hipStream_t st, double* out) {
 }
 
 int Gmg::setup(const double* E, const double* fine_diag, hipStream_t st) {
+  drop_graph();
   const long ne = (long)ctx->g.nx * ctx->g.ny * ctx->g.nz;
   const long n0 = lv[0].n_free;
   GF_HIP(hipMemcpyAsync(E64, E, sizeof(double) * ne, hipMemcpyDeviceToDevice, st));

@@ -46,6 +46,7 @@ class gf_gmg_params(C.Structure):
         ("inner_krylov_steps", C.c_int32),
         ("inner_krylov_restart", C.c_int32),
         ("coarsest_solve", C.c_int32),
+        ("graph_apply", C.c_int32),
     ]
 
 

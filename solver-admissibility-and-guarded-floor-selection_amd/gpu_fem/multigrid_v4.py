@@ -57,7 +57,8 @@ class GalerkinMatFreeGMG:
                  root_local_correction_max_scale: float = 1.0,
                  root_local_correction_inner_steps: int = 10,
                  root_local_correction_inner_restart=None, inner_krylov_steps: int = 0,
-                 inner_krylov_restart=None, coarsest_solve: str = "inverse") -> None:
+                 inner_krylov_restart=None, coarsest_solve: str = "inverse",
+                 graph_apply: bool = True) -> None:
         # argument validation (multigrid_v4.py:600-699), same messages
         if smoother_type not in {"chebyshev", "jacobi"}:
             raise ValueError(f"smoother_type must be 'chebyshev' or 'jacobi'; got {smoother_type!r}")
@@ -160,6 +161,8 @@ class GalerkinMatFreeGMG:
         # applied as one GEMV ("inverse"), or the reference's two triangular
         # solves ("triangular"); both from the same Cholesky factor
         p.coarsest_solve = 0 if coarsest_solve == "inverse" else 1
+        # record the whole application as a hipGraph after each set-up
+        p.graph_apply = 1 if graph_apply else 0
         self._params = p
         h = C.c_void_p()
         _lib.check(lib.gf_gmg_create(C.byref(h), mf_op.ctx_handle, C.byref(p)), "gf_gmg_create")
