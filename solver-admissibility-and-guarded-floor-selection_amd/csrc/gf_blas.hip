@@ -48,15 +48,17 @@ __device__ __forceinline__ void finish_reduce(const double (&v)[NP], double* par
   double bs[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) bs[p] = block_sum(v[p], lds);
-  // partials and ticket live in uncached device memory: a store completes at
-  // the memory, so a work-group release (wait for the store) orders it before
-  // the ticket increment without writing back the whole L2.
+  // partials and ticket live in uncached device memory, so a partial is
+  // visible to every XCD once its store is acknowledged.  Waiting for that
+  // acknowledgement (vmcnt 0) before the ticket increment is the part of an
+  // agent-scope release that matters here; the L2 write-back that such a
+  // release would add is not needed for memory that L2 does not hold.
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int p = 0; p < NP; ++p)
       __hip_atomic_store(&partials[p * kRedBlocks + blockIdx.x], bs[p], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    store_ack_wait();
     const int t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (t == (int)gridDim.x - 1);
   }
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax(long n, const double* __restr
     __hip_atomic_store(&partials[blockIdx.x], smax[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&partials[kRedBlocks + blockIdx.x], smin[0], __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    store_ack_wait();
     last = (__hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
             (int)gridDim.x - 1);
   }

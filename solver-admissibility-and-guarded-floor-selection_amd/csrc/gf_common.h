@@ -79,6 +79,15 @@ struct DofMap {
   }
 };
 
+// Wait until every memory operation this lane issued has been acknowledged
+// (s_waitcnt with all counters at zero; on gfx9 stores count in vmcnt).  The
+// signal fences keep the compiler from moving memory accesses across it.
+__device__ __forceinline__ void store_ack_wait() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Device scalar slots (FP64) used by device-resident control flow.
 struct Scalars {
   double* s;   // base pointer of a device array of doubles
