@@ -13,17 +13,22 @@ from solver_v2.py (62-149, 716-810).  Pinning (``tests/test_oracle_gmg.py``):
                                 P_loc^T KE P_loc restatement for odd dims.
 * ``estimate_*_lambda_max``     bounded above by, and converging to, the
                                 largest eigenvalue of D^-1 K (scipy eigsh/eigh).
-* ``chebyshev_steps``           on a diagonal operator the error is multiplied
-                                by the closed-form Chebyshev residual
-                                polynomial of the recurrence.
-* ``GalerkinMatFreeGMG``        V-cycle with exact coarsest solve on a
-                                2-level hierarchy is a fixed linear operator
-                                (checked by linearity); with 1 level it equals
-                                the smoother; the safeguarded line search
-                                never increases the level residual.
+* ``chebyshev_steps``           degree 1 equals x0 + D^-1 r0 / sigma; with a
+                                zero-width interval on D^-1 A = lam I it is
+                                exact after one step.  The later recurrence
+                                steps follow multigrid_v4.py:1804-1810 as
+                                written (their first direction is the
+                                unscaled z), which is not the textbook
+                                polynomial: pinned end to end only.
+* ``GalerkinMatFreeGMG``        two-level V-cycle with exact coarsest solve and
+                                no line search: linear, symmetric, and B K has
+                                real eigenvalues <= 1 with exactly n_coarse of
+                                them equal to 1 (Galerkin projector); the
+                                safeguarded line search never increases the
+                                level residual and takes the clipped minimiser.
   The nonlinear canonical stack as a whole is pinned end-to-end by the
-  reference data tables (decisions / iteration counts of the policy), see
-  ``tests/test_oracle_policy.py``.  Bitwise intermediate values of the
+  reference's optimized-density row (C64_MF: 17 iterations, final residual
+  within 1e-7 of the table, ``tests/test_oracle_anchor.py``, slow).  Bitwise intermediate values of the
   reference (per-level lambda estimates, accept/reject sequences) cannot be
   pinned without executing the reference: parity unpinned at that
   granularity.

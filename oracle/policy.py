@@ -10,10 +10,11 @@ Restates run_gmg_floor_detector_prospective.py:35-90 and 191-447.  Pinning
   72-state cantilever and 30-state bridge held-out suites (exact integers).
 * ``classify``: recomputes the ``trigger`` column of every reference
   summary row from its own probe_r50 / probe_r100 columns (truth table).
-* ``run_policy``: ladder/guard bookkeeping against the reference rows
-  (attempted floors consistent with trigger + solve_converged) and, on tiny
-  grids, against scripted fake solvers whose projected residual lies (the
-  guard must reject and the ladder must advance).
+* ``run_policy``: scripted solvers whose projected residual lies (the guard
+  must reject and the ladder advance), short probe histories (r50/r100 index
+  clamp), and the raised ladder stopping at the first guarded floor; the
+  reference rows' own attempted-floor / fallback / false-keep columns are
+  checked for the same ladder rules.
 """
 
 from __future__ import annotations
