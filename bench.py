@@ -30,6 +30,12 @@ for p in (ROOT, PKG):
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per FP64 matvec launch on the 80x40x20 grid, from rocprofv3 --pmc
+# FETCH_SIZE (1335.69 KiB) and WRITE_SIZE (1614.59 KiB) per dispatch, each
+# scaled by what the counter reads for a 64 MiB device copy (FETCH 32779.5 KiB
+# -> x1.99927; WRITE 65536 KiB -> x1.0): 2.73 MB read + 1.65 MB written
+# against 2.17 + 1.65 MB algorithmic (profiles/fine_matvec_fp64_80x40x20_hbm_traffic.md)
+TRAFFIC_BYTES = 4387805
 
 
 def parse():
@@ -44,6 +50,8 @@ def parse():
     ap.add_argument("--seeds", default=",".join(str(s) for s in range(32)))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--states-csv", default="")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="time only the FP64 fine matvec loop (for rocprofv3 runs)")
     return ap.parse_args()
 
 
@@ -88,6 +96,36 @@ def matvec_roofline(runner, n_launch=200):
     gbs = bytes_alg / (avg_us * 1e-6) / 1e9
     tfs = flops / (avg_us * 1e-6) / 1e12
     return avg_us, bytes_alg, flops, gbs, tfs
+
+
+def bf16_matvec_rate(runner, n_launch=200):
+    """Average duration of the BF16 matrix-core fused matvec (element
+    contraction + node sum) on the full DOF layout of the same grid."""
+    import torch
+
+    from gpu_fem.cuda_fused_matvec import FusedMatvec
+
+    mf = runner.mf_op
+    fm = FusedMatvec(mf._edof_gpu, mf._KE_unit, mf._ndof, grid=mf.grid)
+    E = torch.rand(mf._n_elem, dtype=torch.float32, device="cuda") + 1e-3
+    u = torch.randn(mf._ndof, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream()
+    for _ in range(5):
+        fm.matvec_full(u, E, dtype="bf16")
+    st.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    u_full, y_full = fm._scratch(u.device)
+    u_full.copy_(u)
+    t0.record(st)
+    for _ in range(n_launch):
+        fm._launch("bf16", u_full, y_full, E)
+    t1.record(st)
+    st.synchronize()
+    us = t0.elapsed_time(t1) * 1e3 / n_launch
+    flops = 1152 * mf._n_elem
+    return {"kernel": "k_elem_bf16 + k_node_gather_f32 (gf_matvec_full_bf16)", "avg_us": us,
+            "bf16_tflops": flops / (us * 1e-6) / 1e12}
 
 
 def cpu_baseline(args, dims, gpu_rows):
@@ -158,6 +196,20 @@ def main():
         dims, load=args.load, penal=args.penal, raised_rho_mins=(1e-3, 1e-2), probe_iters=100,
         probe_r50=50, restart=300, maxiter=300, tol=1e-6)
     gen = states(args)
+    if args.roofline_only:
+        avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner, n_launch=2000)
+        # calibration for HBM counters: a plain device copy of known size
+        # (torch's copy kernel), 64 MiB read + 64 MiB written per launch
+        src = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(50):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel": "gf_matvec_f64", "avg_launch_us": avg_us,
+                          "algorithmic_bytes": bytes_alg, "flops": flops, "GB_s": gbs,
+                          "fp64_tflops": tfs, "launch_shape": runner.mf_op.launch_shape,
+                          "copy_calibration_bytes": {"read": 8 << 23, "write": 8 << 23}}))
+        return
 
     def one():
         seed, p = next(gen)
@@ -179,6 +231,7 @@ def main():
     value = accepted / wall
 
     avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner)
+    bf16 = bf16_matvec_rate(runner)
 
     if args.states_csv:
         import csv
@@ -221,20 +274,24 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "k_fine_matvec<double, EPI_PLAIN> (gf_matvec_f64)",
-            "bound": "hbm",
-            "achieved": gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "kernel": "k_fine_matvec_wave4<double, EPI_PLAIN> (gf_matvec_f64)",
+            "bound": "mfma",
+            "achieved": tfs,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": tfs / FP64_PEAK_TFLOPS,
+            "traffic": TRAFFIC_BYTES,
+            "note": ("FP64 FMA on the vector ALUs; the kernel's 19.3 flop/B is above the FP64 "
+                     "ridge (9.8 flop/B) so compute binds; MI355X FP64 vector and matrix peaks "
+                     "are both 78.6 TFLOP/s"),
             "avg_launch_us": avg_us,
             "algorithmic_bytes": bytes_alg,
             "flops": flops,
-            "fp64_tflops": tfs,
-            "fp64_frac": tfs / FP64_PEAK_TFLOPS,
+            "hbm_GB_s": gbs,
+            "hbm_frac": gbs / HBM_PEAK_GBS,
             "launch_shape": runner.mf_op.launch_shape,
             "autotune_us": runner.mf_op.launch_timings_us,
+            "bf16_mfma_form": bf16,
         },
         "cpu_baseline": cpu,
     }

@@ -12,7 +12,8 @@
  * Reference interfaces replaced (paths relative to the reference repo):
  *   gf_ctx_*            MatrixFreeKff.__init__            src/gpu_fem/solver_v2.py:1153-1175
  *   gf_matvec_f64/f32   MatrixFreeKff.matvec              src/gpu_fem/solver_v2.py:1182-1222
- *   gf_matvec_f32, gf_matvec_full_f32  FusedMatvec.matvec / .matvec_full  cuda_fused_matvec.py:343-391
+ *   gf_matvec_f32, gf_matvec_full_f32, gf_matvec_full_bf16
+ *                       FusedMatvec.matvec / .matvec_full  cuda_fused_matvec.py:343-416
  *   gf_diag_f64         MatrixFreeKff.extract_diagonal    src/gpu_fem/solver_v2.py:1228-1251
  *   gf_nodeblock_*      NodeBlockJacobiMF.update / apply   src/gpu_fem/solver_v2.py:1381-1451
  *   gf_gmg_create       GalerkinMatFreeGMG.__init__       src/gpu_fem/multigrid_v4.py:546-770
@@ -128,6 +129,13 @@ int32_t gf_residual_f64(gf_ctx* ctx, const double* E, const double* b, const dou
  * cuda_fused_matvec.py:379-391). */
 int32_t gf_matvec_full_f32(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
                            void* stream);
+/* BF16 matrix-core form of gf_matvec_full_f32 (FusedMatvec dtype="bf16",
+ * cuda_fused_matvec.py:112-246, 393-416): KE and u rounded to bf16, products
+ * accumulated in FP32 on v_mfma_f32_16x16x32_bf16, E applied in FP32,
+ * element rows summed per node in a fixed order (deterministic).  Allocates
+ * a 24-float-per-element scratch in ctx on first use. */
+int32_t gf_matvec_full_bf16(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
+                            void* stream);
 int32_t gf_diag_f64(gf_ctx* ctx, const double* E, double* diag, void* stream);
 int32_t gf_nodeblock_update(gf_ctx* ctx, const double* E, const double* scalar_diag /*nullable*/,
                             void* stream);

@@ -117,6 +117,15 @@ int32_t gf_matvec_full_f32(gf_ctx* ctx, const float* E, const float* u_full, flo
                             EpiArgs<float>{}, nullptr, S(stream));
 }
 
+int32_t gf_matvec_full_bf16(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
+                            void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  CHECK_PTR(u_full);
+  CHECK_PTR(y_full);
+  return matvec_full_bf16(ctx->c, E, u_full, y_full, S(stream));
+}
+
 int32_t gf_diag_f64(gf_ctx* ctx, const double* E, double* diag, void* stream) {
   CHECK_PTR(ctx);
   CHECK_PTR(E);

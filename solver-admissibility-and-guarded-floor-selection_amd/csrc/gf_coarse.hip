@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
   if (gate && *gate == 0) return;
   __shared__ T part[8][27][3];
   const int k = threadIdx.x & 31, ln = threadIdx.x >> 5;
-  const int n = blockIdx.x * 8 + ln;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * 8 + ln;
   const bool valid = n < g.n_nodes;
   if (valid && k < 27) {
     int ix, iy, iz;
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid g
                                                     const double* __restrict__ ec,
                                                     double* __restrict__ yf, const int* gate) {
   if (gate && *gate == 0) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= gf.n_nodes) return;
   const int o0 = mapf(3 * n), o1 = mapf(3 * n + 1), o2 = mapf(3 * n + 2);
   if ((o0 & o1 & o2) < 0) return;
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
   if (gate && *gate == 0) return;
   __shared__ double part[3][3][64];
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int n = blockIdx.x * 64 + lane;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
   const bool valid = n < gc.n_nodes;
   double acc[3] = {0.0, 0.0, 0.0};
   int o[3] = {-1, -1, -1};

@@ -79,6 +79,29 @@ struct DofMap {
   }
 };
 
+// local node index of position (px,py,pz) in the reference hex ordering
+// (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1)
+__host__ __device__ constexpr int lidx(int px, int py, int pz) {
+  return pz * 4 + (py ? (px ? 2 : 3) : (px ? 1 : 0));
+}
+__host__ __device__ constexpr int lpos(int j, int axis) {
+  // inverse of lidx
+  return axis == 2 ? (j >> 2)
+                   : (axis == 1 ? (((j & 3) == 2 || (j & 3) == 3) ? 1 : 0)
+                                : (((j & 3) == 1 || (j & 3) == 2) ? 1 : 0));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of MI355X (block b
+// runs on XCD b % 8) and each XCD has its own L2.  xcd_tile(b, nb) renumbers
+// blocks so that the blocks of one XCD take one contiguous range of tiles:
+// neighbouring tiles then share an L2 and a stencil's halo is fetched from
+// HBM once per XCD range instead of once per tile.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb % kXcd, k = b % kXcd;
+  return k * q + (k < r ? k : r) + b / kXcd;
+}
+
 // Wait until every memory operation this lane issued has been acknowledged
 // (s_waitcnt with all counters at zero; on gfx9 stores count in vmcnt).  The
 // signal fences keep the compiler from moving memory accesses across it.

@@ -13,18 +13,6 @@
 
 namespace gf {
 
-// local node index of position (px,py,pz) in the reference hex ordering
-// (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1)
-__host__ __device__ constexpr int lidx(int px, int py, int pz) {
-  return pz * 4 + (py ? (px ? 2 : 3) : (px ? 1 : 0));
-}
-__host__ __device__ constexpr int lpos(int j, int axis) {
-  // inverse of lidx
-  return axis == 2 ? (j >> 2)
-                   : (axis == 1 ? (((j & 3) == 2 || (j & 3) == 3) ? 1 : 0)
-                                : (((j & 3) == 1 || (j & 3) == 2) ? 1 : 0));
-}
-
 template <typename T>
 struct Acc {
   using type = T;
@@ -94,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
                                                              const int* gate) {
   using A = typename Acc<T>::type;
   if (gate && *gate == 0) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= g.n_nodes) return;
   const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
   if ((o0 & o1 & o2) < 0) return;
@@ -188,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int a = wv >> 1, b = wv & 1;
-  const int n = blockIdx.x * 64 + lane;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
   const bool valid = n < g.n_nodes;
   const int nn = valid ? n : 0;
   const int iz = nn % g.NZ, t = nn / g.NZ, iy = t % g.NY, ix = t / g.NY;
@@ -321,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_diag(Grid g, DofMap map,
                                                       const double* __restrict__ Ke,
                                                       const double* __restrict__ E,
                                                       double* __restrict__ diag) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= g.n_nodes) return;
   const int iz = n % g.NZ, t = n / g.NZ, iy = t % g.NY, ix = t / g.NY;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -449,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_nodeblock_apply(Grid g, DofMap map,
                                                             double* __restrict__ z,
                                                             const int* gate) {
   if (gate && *gate == 0) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= g.n_nodes) return;
   const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
   if ((o0 >= 0) & (o1 >= 0) & (o2 >= 0)) {

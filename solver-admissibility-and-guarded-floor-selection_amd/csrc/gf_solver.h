@@ -26,6 +26,10 @@ struct Ctx {
   double* nb_inv9 = nullptr;
   double* nb_pinv = nullptr;
   int nb_has_pinv = 0;
+  // bf16 matrix-core contraction (allocated on first use)
+  unsigned short* KEb = nullptr;  // 32x32 bf16 bits, zero padded
+  float* Fel = nullptr;           // 24 floats per element
+  int bf16_alloc();
   DofMap map() const { return DofMap{off, d_f2f}; }
   int init(int nelx, int nely, int nelz, const double* Ke_host, const int* free_host, int nfree,
            int ndof);
@@ -34,6 +38,9 @@ struct Ctx {
   int autotune(hipStream_t st, float* us_out);
   void release();
 };
+
+// y_full = sum_e E_e KE_bf16 bf16(u_e) over all DOFs (FP32 accumulation)
+int matvec_full_bf16(Ctx& c, const float* E, const float* u_full, float* y_full, hipStream_t st);
 
 // One device-resident flexible GMRES instance (fixed maximum restart).
 struct Krylov {

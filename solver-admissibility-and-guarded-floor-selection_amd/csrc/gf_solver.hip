@@ -404,6 +404,10 @@ void Ctx::release() {
   dfree(d_f2f);
   dfree(Ke);
   dfree(Ke32);
+  if (KEb) hipFree(KEb);
+  if (Fel) hipFree(Fel);
+  KEb = nullptr;
+  Fel = nullptr;
   dfree(nb_inv9);
   dfree(nb_pinv);
   work.release();

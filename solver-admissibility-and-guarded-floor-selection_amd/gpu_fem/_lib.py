@@ -85,6 +85,7 @@ SIGNATURES = {
     "gf_matvec_f32": [_P, _P, _P, _P, _P],
     "gf_residual_f64": [_P, _P, _P, _P, _P, _P],
     "gf_matvec_full_f32": [_P, _P, _P, _P, _P],
+    "gf_matvec_full_bf16": [_P, _P, _P, _P, _P],
     "gf_diag_f64": [_P, _P, _P, _P],
     "gf_nodeblock_update": [_P, _P, _P, _P],
     "gf_nodeblock_apply": [_P, _P, _P, _P],

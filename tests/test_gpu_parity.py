@@ -326,3 +326,43 @@ def test_graph_replay_matches_eager():
             _ = ge.apply_fine_operator(r)          # unrelated eager work between replays
             ze = _n(ge.apply(r))
             assert np.array_equal(zg, ze)
+
+
+@pytest.mark.parametrize("dims", [(8, 4, 4), (9, 5, 3), (33, 17, 9)])
+def test_fused_matvec_fp32_and_bf16(dims):
+    """FusedMatvec (cuda_fused_matvec.py:260-416): the FP32 form equals the
+    assembled FP64 K u to FP32 rounding; the BF16 matrix-core form equals the
+    assembled K built from bf16-rounded KE applied to bf16-rounded u (rounding
+    by torch's own bfloat16 conversion), up to FP32 accumulation."""
+    import torch
+    from gpu_fem.cuda_fused_matvec import FusedMatvec
+    from oracle import fem
+
+    spec = fem.cantilever_spec(*dims)
+    bc, free, edof, F, mf = fem.build_components(spec)
+    ndof = bc.ndof
+    rng = np.random.default_rng(11)
+    E = np.where(rng.random(edof.shape[0]) < 0.5, 1.0, 1e-3).astype(np.float32)
+    u = np.zeros(ndof, dtype=np.float32)
+    u[free] = rng.standard_normal(len(free)).astype(np.float32)
+    fm = FusedMatvec(_t(edof.astype(np.int32)), _t(fem.KE_UNIT_3D), ndof)
+
+    K64 = fem.assemble_K(edof, E.astype(np.float64), fem.KE_UNIT_3D, ndof)
+    y32 = _n(fm.matvec_full(_t(u), _t(E), dtype="fp32"))
+    ref = K64 @ u.astype(np.float64)
+    assert np.abs(y32 - ref).max() <= 2e-6 * np.abs(ref).max()
+
+    def bf(a):
+        return torch.as_tensor(np.asarray(a, dtype=np.float32)).to(torch.bfloat16).to(
+            torch.float64).numpy()
+
+    KEb = bf(fem.KE_UNIT_3D.astype(np.float32))
+    Kb = fem.assemble_K(edof, E.astype(np.float64), KEb, ndof)
+    refb = Kb @ bf(u)
+    yb = _n(fm.matvec_full(_t(u), _t(E), dtype="bf16"))
+    assert np.abs(yb - refb).max() <= 2e-6 * np.abs(refb).max()
+    # bf16 rounding error itself is ~1e-3 relative: the form is not the FP32 one
+    assert np.abs(yb - ref).max() <= 2e-2 * np.abs(ref).max()
+    # free-DOF layout wrapper
+    yf = _n(fm.matvec(_t(u[free]), _t(E), _t(free.astype(np.int64)), dtype="bf16"))
+    assert np.array_equal(yf, yb[free])
