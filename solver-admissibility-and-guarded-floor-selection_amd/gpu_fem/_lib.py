@@ -11,7 +11,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libgfhip.so")
+LIB_PATH = os.environ.get("GFHIP_LIB") or os.path.join(_HERE, "_lib", "libgfhip.so")
 
 _lib = None
 
