@@ -41,14 +41,19 @@ TRAFFIC_BYTES = 4387805
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=9)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--grid", default="80x40x20")
     ap.add_argument("--penal", type=float, default=3.0)
     ap.add_argument("--load", default="edge", choices=["edge", "tip"])
     ap.add_argument("--probabilities", default="0.2,0.35,0.5")
     ap.add_argument("--seeds", default=",".join(str(s) for s in range(32)))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workers", type=int, default=4,
+                    help="solver instances running states concurrently (one session)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="HIP hardware queues for this process (GPU_MAX_HW_QUEUES, <= 32; "
+                         "0 keeps the runtime default)")
     ap.add_argument("--states-csv", default="")
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the FP64 fine matvec loop (for rocprofv3 runs)")
@@ -181,6 +186,11 @@ def cpu_baseline(args, dims, gpu_rows):
 
 def main():
     args = parse()
+    if args.hw_queues:
+        if not 1 <= args.hw_queues <= 32:
+            raise SystemExit("--hw-queues must be in [1, 32]")
+        # read by the HIP runtime at initialisation, which happens below
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(states shard across GPUs with no communication; "
@@ -192,9 +202,14 @@ def main():
     from gpu_fem import policy as gpol
 
     dims = tuple(int(v) for v in args.grid.split("x"))
-    runner = gpol.GuardedFloorPolicy.from_cantilever(
-        dims, load=args.load, penal=args.penal, raised_rho_mins=(1e-3, 1e-2), probe_iters=100,
-        probe_r50=50, restart=300, maxiter=300, tol=1e-6)
+    from gpu_fem.presets import cantilever
+
+    spec = cantilever(*dims, load=args.load)
+    kw = dict(penal=args.penal, raised_rho_mins=(1e-3, 1e-2), probe_iters=100, probe_r50=50,
+              restart=300, maxiter=300, tol=1e-6,
+              preset_name=f"cantilever_{dims[0]}x{dims[1]}x{dims[2]}_{args.load}")
+    session = gpol.PolicySession(spec, workers=args.workers, **kw)
+    runner = session.runners[0]
     gen = states(args)
     if args.roofline_only:
         avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner, n_launch=2000)
@@ -210,21 +225,30 @@ def main():
                           "fp64_tflops": tfs, "launch_shape": runner.mf_op.launch_shape,
                           "copy_calibration_bytes": {"read": 8 << 23, "write": 8 << 23}}))
         return
+    n_e = dims[0] * dims[1] * dims[2]
 
-    def one():
-        seed, p = next(gen)
-        mask = np.random.default_rng(seed).random(dims[0] * dims[1] * dims[2]) < p
-        row = runner.run_state(mask, seed=seed, probability=p, keep_x=False)
+    def batch(k):
+        out = []
+        for _ in range(k):
+            seed, p = next(gen)
+            mask = np.random.default_rng(seed).random(n_e) < p
+            out.append({"solid": mask, "seed": seed, "probability": p})
+        return out
+
+    def finish(row):
         row["total_iters"] = (len(row["probe_history"]) - 1 +
                               sum(len(h) - 1 for _, h in row["solve_history"]))
         row["n_setups"] = 1 + row["n_floor_attempts"]
         return row
 
-    for _ in range(args.warmup):
-        one()
+    # warm-up: the session's first state (sets the cached lambda for every
+    # instance) and then one state per instance
+    if args.warmup > 0:
+        session.run(batch(max(args.warmup, args.workers)))
     torch.cuda.synchronize()
+    timed = batch(args.steps)
     t0 = time.perf_counter()
-    rows = [one() for _ in range(args.steps)]
+    rows = [finish(r) for r in session.run(timed)]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     accepted = sum(int(r["solve_converged"]) for r in rows)
@@ -271,7 +295,7 @@ def main():
             "triggers": [r["trigger"] for r in rows],
             "selected_floors": [r["recommended_rho_min"] for r in rows],
             "max_true_rel_residual": float(max(r["solve_final_rel_residual"] for r in rows)),
-            "parallelism": "single GPU",
+            "parallelism": f"single GPU, {args.workers} concurrent solver instances (one session)",
         },
         "roofline": {
             "kernel": "k_fine_matvec_wave4<double, EPI_PLAIN> (gf_matvec_f64)",

@@ -142,8 +142,8 @@ int32_t gf_nodeblock_update(gf_ctx* ctx, const double* E, const double* scalar_d
 int32_t gf_nodeblock_apply(gf_ctx* ctx, const double* r, double* z, void* stream);
 
 /* ---- BLAS-1 (deterministic fixed-order reductions) -------------------- */
-/* result written to a device double (d_out); ctx may be NULL (process-wide
- * scratch, default-stream use only) */
+/* result written to a device double (d_out); ctx may be NULL (a per-thread
+ * scratch: calls from one thread must not overlap on two streams) */
 int32_t gf_dot_f64(gf_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out,
                    void* stream);
 int32_t gf_nrm2_f64(gf_ctx* ctx, int64_t n, const double* x, double* d_out, void* stream);

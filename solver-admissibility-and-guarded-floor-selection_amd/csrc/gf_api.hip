@@ -156,10 +156,11 @@ int32_t gf_nodeblock_apply(gf_ctx* ctx, const double* r, double* z, void* stream
                          ctx->c.nb_has_pinv, r, z, nullptr, S(stream));
 }
 
-// reductions without a context use one process-wide scratch (single stream)
+// reductions without a context use a per-thread scratch (one stream at a
+// time per thread)
 static Work* blas_work(gf_ctx* ctx) {
   if (ctx) return &ctx->c.work;
-  static Work w;
+  static thread_local Work w;
   if (w.init() != 0) return nullptr;
   return &w;
 }

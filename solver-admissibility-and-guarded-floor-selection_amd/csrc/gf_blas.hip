@@ -8,6 +8,8 @@
 // loops use this to stop after a breakdown without host synchronisation.
 #include "gf_internal.h"
 
+#include <mutex>
+
 namespace gf {
 
 std::string& err_slot() {
@@ -376,12 +378,20 @@ int count_gt(Work& w, long n, const double* E, const double* thr_src, double sca
 // kernels have read are kept until the process ends).
 namespace {
 struct UncachedPool {
+  std::mutex mu;
   std::vector<std::pair<double*, int*>> free_list;
 } g_pool;
 }  // namespace
 
+std::recursive_mutex& device_mutex() {
+  static std::recursive_mutex mu;
+  return mu;
+}
+
 int Work::init() {
   if (partials) return 0;
+  GF_DEVICE_LOCK();
+  std::lock_guard<std::mutex> lock(g_pool.mu);
   if (!g_pool.free_list.empty()) {
     partials = g_pool.free_list.back().first;
     ticket = g_pool.free_list.back().second;
@@ -396,6 +406,8 @@ int Work::init() {
 }
 
 void Work::release() {
+  GF_DEVICE_LOCK();
+  std::lock_guard<std::mutex> lock(g_pool.mu);
   if (partials) g_pool.free_list.emplace_back(partials, ticket);
   partials = nullptr;
   ticket = nullptr;

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 namespace gf {
@@ -101,6 +102,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   const int q = nb / kXcd, r = nb % kXcd, k = b % kXcd;
   return k * q + (k < r ? k : r) + b / kXcd;
 }
+
+// Serialises device allocation, freeing and legacy-stream set-up calls with
+// hipGraph capture across host threads: a hipMalloc/hipFree/hipMemset issued
+// by one thread while another thread's stream is capturing can fail (or
+// invalidate the capture), so every such site and every capture holds this
+// lock.  Recursive: release paths run inside allocation paths.
+std::recursive_mutex& device_mutex();
+#define GF_DEVICE_LOCK() std::lock_guard<std::recursive_mutex> _gf_dev_lock(::gf::device_mutex())
 
 // Wait until every memory operation this lane issued has been acknowledged
 // (s_waitcnt with all counters at zero; on gfx9 stores count in vmcnt).  The

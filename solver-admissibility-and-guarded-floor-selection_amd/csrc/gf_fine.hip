@@ -11,6 +11,8 @@
 // the scalar cache and feeds the FMAs as scalar operands.
 #include "gf_internal.h"
 
+#include <atomic>
+
 namespace gf {
 
 template <typename T>
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
   if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
 }
 
-static int g_split_f64 = 4, g_split_f32 = 4;
+static std::atomic<int> g_split_f64{4}, g_split_f32{4};
 
 void set_fine_split(int split_f64, int split_f32) {
   if (split_f64 == 1 || split_f64 == 4) g_split_f64 = split_f64;

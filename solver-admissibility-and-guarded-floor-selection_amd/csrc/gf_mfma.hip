@@ -104,6 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_node_gather_f32(Grid g, const float*
 
 int Ctx::bf16_alloc() {
   if (KEb && Fel) return 0;
+  GF_DEVICE_LOCK();
   unsigned short h[32 * 32];
   for (int i = 0; i < 32 * 32; ++i) h[i] = 0;
   for (int i = 0; i < 24; ++i)

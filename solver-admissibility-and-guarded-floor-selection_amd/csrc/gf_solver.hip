@@ -309,6 +309,7 @@ static void dfree(T*& p) {
 // ---------------------------------------------------------------------------
 int Ctx::init(int nelx, int nely, int nelz, const double* Ke_host, const int* free_host,
               int nfree, int ndof) {
+  GF_DEVICE_LOCK();
   if (nelx < 1 || nely < 1 || nelz < 1) {
     set_error("gf_ctx_create: grid dimensions must be >= 1");
     return -2;
@@ -348,6 +349,7 @@ int Ctx::init(int nelx, int nely, int nelz, const double* Ke_host, const int* fr
 }
 
 int Ctx::nodeblock_alloc() {
+  GF_DEVICE_LOCK();
   if (nb_inv9) return 0;
   GF_TRY(dalloc(&nb_inv9, 9L * g.n_nodes));
   GF_TRY(dalloc(&nb_pinv, n_free));
@@ -401,6 +403,7 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
 }
 
 void Ctx::release() {
+  GF_DEVICE_LOCK();
   dfree(d_f2f);
   dfree(Ke);
   dfree(Ke32);
@@ -417,6 +420,7 @@ void Ctx::release() {
 // Krylov
 // ---------------------------------------------------------------------------
 int Krylov::alloc(int m_, long n_) {
+  GF_DEVICE_LOCK();
   if (V && m_ == m && n_ == n) return 0;
   release();
   m = m_;
@@ -436,6 +440,7 @@ int Krylov::alloc(int m_, long n_) {
 }
 
 void Krylov::release() {
+  GF_DEVICE_LOCK();
   dfree(V);
   dfree(Z);
   dfree(H);
@@ -532,6 +537,7 @@ static int level_prec_code(const gf_gmg_params& p, int l) {
 }
 
 int Gmg::create(Ctx* c, const gf_gmg_params* params) {
+  GF_DEVICE_LOCK();
   ctx = c;
   p = *params;
   if (p.smoother_type < 0 || p.smoother_type > 1 || p.cycle_type < 0 || p.cycle_type > 1 ||
@@ -654,6 +660,7 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
 }
 
 void Gmg::release() {
+  GF_DEVICE_LOCK();
   for (int l = 0; l < (int)lv.size(); ++l) {
     Level& v = lv[l];
     if (l > 0) dfree(v.d_f2f);
@@ -1046,6 +1053,7 @@ void Gmg::drop_graph() {
 // static for a given set-up (all decisions are device-side), so one graph is
 // replayed until the next set-up.  It contains kernels only.
 int Gmg::capture() {
+  GF_DEVICE_LOCK();
   drop_graph();
   if (!cap_stream) GF_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
   GF_HIP(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));

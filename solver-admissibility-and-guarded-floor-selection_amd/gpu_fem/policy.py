@@ -109,6 +109,14 @@ def _build_components(preset_or_spec, *, fused_op=None, **stack):
     return spec, bc, free, free_gpu, edof_gpu, F_free_gpu, mf_op, gmg
 
 
+def _sync():
+    """Wait for this thread's current stream only (other threads may be
+    driving independent solves on their own streams)."""
+    import torch
+
+    torch.cuda.current_stream().synchronize()
+
+
 def _mem_used_mb() -> float:
     import torch
 
@@ -171,17 +179,17 @@ class GuardedFloorPolicy:
 
         E_host = self.moduli(rho_min, solid, rho_field)
         E = torch.as_tensor(E_host, device="cuda")
-        torch.cuda.synchronize()
+        _sync()
         mem_before = _mem_used_mb()
         t0 = time.perf_counter()
         self.gmg.setup(E)
-        torch.cuda.synchronize()
+        _sync()
         setup_time = time.perf_counter() - t0
         mem_setup = _mem_used_mb()
         t1 = time.perf_counter()
         x, iters, reported = self.gmg.solve(self.F, tol=self.tol, maxiter=maxiter,
                                             restart=self.restart, history=history)
-        torch.cuda.synchronize()
+        _sync()
         solve_time = time.perf_counter() - t1
         r = self.mf_op.residual(self.F, x, E)
         final = nrm2(r) / self.b_norm
@@ -282,6 +290,96 @@ class GuardedFloorPolicy:
         if keep_x:
             row["x"] = st["x"]
         return row
+
+
+class PolicySession:
+    """Many states through the guarded policy as ONE reference session, with
+    several solver instances running concurrently.
+
+    The reference runs a multi-state evaluation as one session: one hierarchy
+    whose lambda_max is estimated at the session's first set-up (the first
+    state's probe) and reused while max(E) stays within 10 %
+    (multigrid_v4.py:1170-1192).  Here ``workers`` independent
+    ``GuardedFloorPolicy`` instances each own a HIP stream and are driven from
+    host threads (the C calls release the GIL).  The first state runs alone on
+    instance 0; every other instance then performs the same first set-up on
+    that state's probe moduli, so all instances hold the identical cached
+    lambda.  Every kernel is deterministic, so each state's row equals what a
+    single sequential session would produce, whatever the interleaving.
+    """
+
+    def __init__(self, preset_or_spec, *, workers: int = 4, **kw):
+        import torch
+
+        if workers < 1:
+            raise ValueError("workers must be >= 1")
+        self.runners = [GuardedFloorPolicy(preset_or_spec, **kw) for _ in range(workers)]
+        self.streams = [torch.cuda.Stream() for _ in range(workers)]
+        self._started = False
+
+    @property
+    def workers(self) -> int:
+        return len(self.runners)
+
+    def _run_one(self, k, st):
+        import torch
+
+        with torch.cuda.stream(self.streams[k]):
+            row = self.runners[k].run_state(st.get("solid"), rho_field=st.get("rho_field"),
+                                            seed=st.get("seed"),
+                                            probability=st.get("probability"),
+                                            keep_x=st.get("keep_x", False))
+            _sync()
+        return row
+
+    def run(self, states):
+        """states: list of dicts with ``solid`` (or ``rho_field``) and optional
+        ``seed`` / ``probability`` / ``keep_x``.  Returns the rows in order."""
+        import queue
+        import threading
+
+        import torch
+
+        rows = [None] * len(states)
+        start = 0
+        if not self._started and states:
+            st0 = states[0]
+            rows[0] = self._run_one(0, st0)
+            r0 = self.runners[0]
+            E0 = torch.as_tensor(r0.moduli(r0.baseline_rho_min, st0.get("solid"),
+                                           st0.get("rho_field")), device="cuda")
+            for k in range(1, self.workers):
+                with torch.cuda.stream(self.streams[k]):
+                    self.runners[k].gmg.setup(E0)
+                    _sync()
+            self._started = True
+            start = 1
+        todo = queue.Queue()
+        for i in range(start, len(states)):
+            todo.put(i)
+        errors = []
+
+        def worker(k):
+            while not errors:
+                try:
+                    i = todo.get_nowait()
+                except queue.Empty:
+                    return
+                try:
+                    rows[i] = self._run_one(k, states[i])
+                except BaseException as ex:  # surfaced in the caller
+                    errors.append(ex)
+                    return
+
+        threads = [threading.Thread(target=worker, args=(k,), daemon=True)
+                   for k in range(min(self.workers, max(1, len(states) - start)))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        return rows
 
 
 SUMMARY_FIELDS = [

@@ -11,6 +11,7 @@ arrays); all arithmetic runs in libgfhip.so.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 
 import numpy as np
 
@@ -204,16 +205,21 @@ class NodeBlockJacobiMF:
 # ---------------------------------------------------------------------------
 # BLAS-1 helpers on libgfhip (deterministic reductions)
 # ---------------------------------------------------------------------------
-_scalar_cache = {}
+_scalar_tls = threading.local()
 
 
 def _scalar_buf(device):
+    """A 2-double device buffer for scalar results, one per thread and device
+    (threads drive independent solves on their own streams)."""
     import torch
 
+    cache = getattr(_scalar_tls, "bufs", None)
+    if cache is None:
+        cache = _scalar_tls.bufs = {}
     key = str(device)
-    if key not in _scalar_cache:
-        _scalar_cache[key] = torch.zeros(2, dtype=torch.float64, device=device)
-    return _scalar_cache[key]
+    if key not in cache:
+        cache[key] = torch.zeros(2, dtype=torch.float64, device=device)
+    return cache[key]
 
 
 def dot(x, y) -> float:

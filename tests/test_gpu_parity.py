@@ -366,3 +366,30 @@ def test_fused_matvec_fp32_and_bf16(dims):
     # free-DOF layout wrapper
     yf = _n(fm.matvec(_t(u[free]), _t(E), _t(free.astype(np.int64)), dtype="bf16"))
     assert np.array_equal(yf, yb[free])
+
+
+def test_concurrent_session_equals_sequential():
+    """PolicySession runs states on several solver instances at once; every
+    row (decisions, histories, solution) equals the single-instance session's
+    bit for bit."""
+    from gpu_fem import policy as gpol
+    from gpu_fem.presets import cantilever
+
+    dims = (16, 8, 8)
+    states = [(s, p) for s in (3, 4) for p in (0.2, 0.35, 0.5)] + [(5, 0.15)]
+
+    def mk(s, p):
+        return np.random.default_rng(s).random(int(np.prod(dims))) < p
+
+    seq = gpol.GuardedFloorPolicy(cantilever(*dims), penal=4.5)
+    rows_seq = [seq.run_state(mk(s, p), seed=s, probability=p) for s, p in states]
+    sess = gpol.PolicySession(cantilever(*dims), workers=3, penal=4.5)
+    rows_par = sess.run([dict(solid=mk(s, p), seed=s, probability=p, keep_x=True)
+                         for s, p in states])
+    for (s, p), a, b in zip(states, rows_seq, rows_par):
+        for k in ("trigger", "attempted_rho_mins", "recommended_rho_min", "solve_iters",
+                  "solve_converged", "probe_r50", "probe_r100", "solve_final_rel_residual",
+                  "compliance"):
+            assert a[k] == b[k], (s, p, k, a[k], b[k])
+        assert a["probe_history"] == b["probe_history"], (s, p)
+        assert np.array_equal(_n(a["x"]), _n(b["x"])), (s, p)
