@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workers", type=int, default=4,
                     help="solver instances running states concurrently (one session)")
+    ap.add_argument("--mgs-sweep", type=int, default=1, choices=[0, 1, 2],
+                    help="outer FGMRES orthogonalisation: 1 persistent sweep, 2 cooperative "
+                         "launch, 0 one launch per projection")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="HIP hardware queues for this process (GPU_MAX_HW_QUEUES, <= 32; "
                          "0 keeps the runtime default)")
@@ -206,7 +209,7 @@ def main():
 
     spec = cantilever(*dims, load=args.load)
     kw = dict(penal=args.penal, raised_rho_mins=(1e-3, 1e-2), probe_iters=100, probe_r50=50,
-              restart=300, maxiter=300, tol=1e-6,
+              restart=300, maxiter=300, tol=1e-6, mgs_sweep=args.mgs_sweep,
               preset_name=f"cantilever_{dims[0]}x{dims[1]}x{dims[2]}_{args.load}")
     session = gpol.PolicySession(spec, workers=args.workers, **kw)
     runner = session.runners[0]

@@ -79,6 +79,14 @@ typedef struct gf_gmg_params {
   /* 1 = record the whole preconditioner application as a hipGraph after each
    * set-up and replay it (default); 0 = launch kernel by kernel */
   int32_t graph_apply;
+  /* outer FGMRES orthogonalisation (gf_gmg_solve): 1 = the whole modified
+   * Gram-Schmidt sweep of an Arnoldi step in one persistent launch with the
+   * vector held in registers and bounded grid-wide waits (default; falls
+   * back when n is too large; also used by the inner and root-local FGMRES
+   * recorded in the preconditioner graph), 2 = as 1 but the outer sweeps go
+   * through hipLaunchCooperativeKernel, 0 = one launch per projection.  Same
+   * arithmetic order in every mode up to the partial-sum grouping. */
+  int32_t mgs_sweep;
 } gf_gmg_params;
 
 /* State read back after gf_gmg_setup (properties of GalerkinMatFreeGMG,

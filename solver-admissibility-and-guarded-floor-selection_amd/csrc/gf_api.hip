@@ -228,6 +228,7 @@ void gf_gmg_default_params(gf_gmg_params* p) {
   p->inner_krylov_restart = 0;
   p->coarsest_solve = 0;
   p->graph_apply = 1;
+  p->mgs_sweep = 1;
 }
 
 int32_t gf_gmg_create(gf_gmg** out, gf_ctx* ctx, const gf_gmg_params* params) {

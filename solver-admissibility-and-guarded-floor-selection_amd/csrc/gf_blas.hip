@@ -8,6 +8,7 @@
 // loops use this to stop after a breakdown without host synchronisation.
 #include "gf_internal.h"
 
+#include <atomic>
 #include <mutex>
 
 namespace gf {
@@ -141,6 +142,183 @@ __global__ __launch_bounds__(kBlock) void k_mgs_axpy_dot(long n, const double* _
     a[0] += (next ? next[i] : wv) * wv;
   }
   finish_reduce<1>(a, partials, ticket, out, post);
+}
+
+
+// ---------------------------------------------------------------------------
+// Modified Gram-Schmidt sweep of one Arnoldi step in one persistent launch.
+// w stays in registers (PER elements per thread); each projection needs one
+// grid-wide exchange of block partials.  A block publishes its partial in
+// uncached memory, waits for the store's acknowledgement, then raises its
+// arrival flag to base+s+1; every block polls all flags and then folds the
+// partials in block order, so every block derives the same h_s bit for bit.
+// Partials alternate between two slots per block: a block can only overwrite
+// slot s&1 after every block has raised its flag for step s+1, i.e. after
+// every block has finished reading step s.
+// ---------------------------------------------------------------------------
+constexpr int kSweepBlock = 256;
+constexpr long kSweepSpinCap = 1L << 20;
+
+template <int PER>
+__global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
+                                                          const double* __restrict__ V,
+                                                          double* __restrict__ w, double* H,
+                                                          double* part, int* flags, double* err,
+                                                          const int* gate) {
+  if (gate && *gate == 0) return;  // uniform: written by an earlier kernel
+  __shared__ double lds[4];
+  __shared__ double hsh;
+  __shared__ unsigned f0s;
+  const int G = gridDim.x, b = blockIdx.x;
+  // Every complete sweep raises every block's flag by j+2, so all flags are
+  // equal when a sweep starts; a block's own flag is the common base.  Flags
+  // are compared with wrap-around (serial-number) arithmetic.
+  if (threadIdx.x == 0)
+    f0s = (unsigned)__hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const unsigned f0 = f0s;
+  const long stride = (long)G * kSweepBlock;
+  const long base = (long)b * kSweepBlock + threadIdx.x;
+  double wr[PER], vr[PER], vn[PER];
+  bool broken = false;  // wave 0 only
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const long i = base + k * stride;
+    wr[k] = i < n ? w[i] : 0.0;
+    vr[k] = i < n ? V[i] : 0.0;
+  }
+  for (int s = 0; s <= j + 1; ++s) {
+    double a = 0.0;
+    if (s <= j) {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) a += wr[k] * vr[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) a += wr[k] * wr[k];
+    }
+    const double bs = block_sum(a, lds);
+    if (s < j) {  // prefetch V_{s+1} while the exchange is in flight
+      const double* vs = V + (long)(s + 1) * n;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const long i = base + k * stride;
+        vn[k] = i < n ? vs[i] : 0.0;
+      }
+    }
+    const int slot = (s & 1) * kSweepMaxBlocks;
+    const unsigned target = f0 + (unsigned)(s + 1);
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&part[slot + b], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      store_ack_wait();
+      __hip_atomic_store(&flags[b], (int)target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x < 64) {
+      // bounded wait; after one expired wait the block stops waiting (the
+      // host sees *err and discards the result), so a sweep always ends
+      long spins = 0;
+      while (!broken) {
+        int ok = 1;
+        for (int q = threadIdx.x; q < G; q += 64)
+          ok &= (int)((unsigned)__hip_atomic_load(&flags[q], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT) -
+                      target) >= 0;
+        if (__all(ok)) break;
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSweepSpinCap) {
+          broken = true;
+          if (threadIdx.x == 0) *err = 1.0;
+        }
+      }
+      double t = 0.0;
+      for (int q = threadIdx.x; q < G; q += 64)
+        t += __hip_atomic_load(&part[slot + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      t = wave_sum(t);
+      if (threadIdx.x == 0) hsh = t;
+    }
+    __syncthreads();
+    const double h = hsh;
+    if (s <= j) {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) wr[k] -= h * vr[k];
+      if (b == 0 && threadIdx.x == 0) H[(long)s * m + j] = h;
+      if (s < j) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) vr[k] = vn[k];
+      }
+    } else {
+      const double nrm = sqrt(h);
+      if (b == 0 && threadIdx.x == 0) H[(long)(j + 1) * m + j] = nrm;
+      const bool scale = nrm > 1e-14;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const long i = base + k * stride;
+        if (i < n) w[i] = scale ? wr[k] / nrm : wr[k];
+      }
+    }
+  }
+}
+
+__global__ void k_fill_int(int n, int* p, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+namespace {
+template <int PER>
+int sweep_fits(int G) {
+  static std::atomic<int> max_blocks{-1};  // co-resident blocks of this instantiation
+  if (max_blocks.load() < 0) {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mgs_sweep<PER>, kSweepBlock, 0) !=
+        hipSuccess)
+      return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    max_blocks.store(per_cu * cus);
+  }
+  return G <= max_blocks.load();
+}
+
+template <int PER>
+int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H,
+                 double* err, const int* gate, int cooperative, int G, hipStream_t st) {
+  if (!sweep_fits<PER>(G)) return 0;
+  if (wk.sweep_G != G) {
+    // the sweep needs all flags of its grid equal at entry; flags left by a
+    // different grid (or an unknown history) are reset first -- a plain
+    // kernel, so this also holds where the sequence is recorded in a graph
+    hipLaunchKernelGGL(k_fill_int, dim3(4), dim3(256), 0, st, kSweepMaxBlocks, wk.flags, 0);
+    GF_CHECK_LAUNCH();
+    wk.sweep_G = G;
+  }
+  if (cooperative) {
+    void* args[] = {&n, &j, &m, (void*)&V, &wv, &H, &wk.partials, &wk.flags, &err, (void*)&gate};
+    GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER>), dim3(G),
+                                      dim3(kSweepBlock), args, 0, st));
+  } else {
+    // G is far below the device's co-resident capacity (checked above), so
+    // every block is resident once the kernels ahead of it drain; the waits
+    // are bounded regardless
+    hipLaunchKernelGGL(k_mgs_sweep<PER>, dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
+                       wk.partials, wk.flags, err, gate);
+    GF_CHECK_LAUNCH();
+  }
+  return 1;
+}
+}  // namespace
+
+int mgs_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H, double* err,
+              const int* gate, int cooperative, hipStream_t st) {
+  constexpr int kPerMax = 16;
+  int G = cdiv(n, (long)kSweepBlock * kPerMax);
+  if (G < 1) G = 1;
+  if (G > kSweepMaxBlocks) return 0;
+  const long per = (n + (long)G * kSweepBlock - 1) / ((long)G * kSweepBlock);
+  if (per <= 1) return launch_sweep<1>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  if (per <= 2) return launch_sweep<2>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  if (per <= 4) return launch_sweep<4>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  if (per <= 8) return launch_sweep<8>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  return launch_sweep<16>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
 }
 
 // y = b - s*x ; out = ||y||  with s = *sptr (trial residual norm, y may be null)
@@ -377,9 +555,14 @@ int count_gt(Work& w, long n, const double* E, const double* thr_src, double sca
 // wide free list and never returned to the driver (uncached allocations that
 // kernels have read are kept until the process ends).
 namespace {
+struct Pooled {
+  double* partials;
+  int* ticket;
+  int* flags;
+};
 struct UncachedPool {
   std::mutex mu;
-  std::vector<std::pair<double*, int*>> free_list;
+  std::vector<Pooled> free_list;
 } g_pool;
 }  // namespace
 
@@ -393,8 +576,11 @@ int Work::init() {
   GF_DEVICE_LOCK();
   std::lock_guard<std::mutex> lock(g_pool.mu);
   if (!g_pool.free_list.empty()) {
-    partials = g_pool.free_list.back().first;
-    ticket = g_pool.free_list.back().second;
+    const Pooled& e = g_pool.free_list.back();
+    partials = e.partials;
+    ticket = e.ticket;
+    flags = e.flags;
+    sweep_G = -1;
     g_pool.free_list.pop_back();
     return 0;
   }
@@ -402,15 +588,20 @@ int Work::init() {
                                hipDeviceMallocUncached));
   GF_HIP(hipExtMallocWithFlags((void**)&ticket, sizeof(int) * 64, hipDeviceMallocUncached));
   GF_HIP(hipMemset(ticket, 0, sizeof(int) * 64));
+  GF_HIP(hipExtMallocWithFlags((void**)&flags, sizeof(int) * kSweepMaxBlocks,
+                               hipDeviceMallocUncached));
+  GF_HIP(hipMemset(flags, 0, sizeof(int) * kSweepMaxBlocks));
+  sweep_G = -1;
   return 0;
 }
 
 void Work::release() {
   GF_DEVICE_LOCK();
   std::lock_guard<std::mutex> lock(g_pool.mu);
-  if (partials) g_pool.free_list.emplace_back(partials, ticket);
+  if (partials) g_pool.free_list.push_back(Pooled{partials, ticket, flags});
   partials = nullptr;
   ticket = nullptr;
+  flags = nullptr;
 }
 
 }  // namespace gf

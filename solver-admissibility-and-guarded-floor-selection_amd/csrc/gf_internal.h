@@ -13,9 +13,15 @@ namespace gf {
 struct Work {
   double* partials = nullptr;  // 4 * kRedBlocks doubles
   int* ticket = nullptr;
+  int* flags = nullptr;        // kSweepMaxBlocks arrival flags of the MGS sweep
+  int sweep_G = -1;            // grid of the last sweep launched on flags (-1: unknown)
   int init();
   void release();
 };
+
+// Blocks of one cooperative MGS sweep (bounded by the arrival-flag array and
+// by the partials, two slots per block).
+constexpr int kSweepMaxBlocks = 1024;
 
 // Epilogue applied by the operator kernels to each output DOF i:
 //   PLAIN : y_i = (A x)_i
@@ -44,6 +50,16 @@ int dot4_ls(Work& w, long n, const double* c, const double* r, const double* Ac,
             const int* gate, hipStream_t st);
 int mgs_axpy_dot(Work& w, long n, const double* v, double* wv, const double* h,
                  const double* next, double* out, bool sqrt_, const int* gate, hipStream_t st);
+// One Arnoldi orthogonalisation by modified Gram-Schmidt in a single
+// cooperative launch: for s = 0..j, h_s = V_s . w, w -= h_s V_s; then
+// h_{j+1} = ||w|| and w /= h_{j+1} when h_{j+1} > 1e-14.  H column j
+// (row-major, leading dimension m) receives h_0..h_{j+1}.  *err is set to 1
+// if a grid barrier wait exceeds its bound.  Skipped when *gate == 0.  Safe to
+// record in a hipGraph (no launch-time state).  Returns 1 when the sweep was
+// launched, 0 when n is outside the register-resident range (the caller then
+// uses mgs_axpy_dot per projection), negative on error.
+int mgs_sweep(Work& w, long n, const double* V, int j, int m, double* wv, double* H, double* err,
+              const int* gate, int cooperative, hipStream_t st);
 int trial_norm(Work& w, long n, const double* b, const double* x, const double* s, double* y,
                double* out, const int* gate, hipStream_t st);
 int axpby_dev(long n, const double* a, double asign, const double* x, const double* b, double* y,

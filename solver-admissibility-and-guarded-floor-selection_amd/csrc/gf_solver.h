@@ -53,7 +53,7 @@ struct Krylov {
   double* sn = nullptr;
   double* g = nullptr;   // m+1
   double* y = nullptr;   // m
-  double* sc = nullptr;  // scalars: [0]=bnorm [1]=beta [2]=hist value
+  double* sc = nullptr;  // scalars: [0]=bnorm [1]=beta [2]=hist value [3]=h_{j+1,j} [4]=sweep error
   int* fl = nullptr;     // [0]=step gate [1]=run gate [2]=jfinal [3]=converged [4]=total [5]=err
   int alloc(int m_, long n_);
   void release();
@@ -120,6 +120,7 @@ struct Gmg {
   // outer solve
   Krylov outer;
   double* hs = nullptr;       // pinned host mirror for outer control
+  double* d_err = nullptr;    // set to 1 by a Gram-Schmidt sweep whose grid-wide wait expired
   // scratch for power iterations
   double *pw_v = nullptr, *pw_d = nullptr, *pw_kv = nullptr, *pw_av = nullptr, *pw_s = nullptr;
   Work work;

@@ -407,10 +407,8 @@ void Ctx::release() {
   dfree(d_f2f);
   dfree(Ke);
   dfree(Ke32);
-  if (KEb) hipFree(KEb);
-  if (Fel) hipFree(Fel);
-  KEb = nullptr;
-  Fel = nullptr;
+  dfree(KEb);
+  dfree(Fel);
   dfree(nb_inv9);
   dfree(nb_pinv);
   work.release();
@@ -644,6 +642,8 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
   GF_TRY(dalloc(&pw_av, n0));
   GF_TRY(dalloc(&pw_s, 1024));
   GF_HIP(hipHostMalloc((void**)&hs, sizeof(double) * 16, hipHostMallocDefault));
+  GF_TRY(dalloc(&d_err, 1));
+  GF_HIP(hipMemset(d_err, 0, sizeof(double)));
   if (p.root_enrichment_mode != 0) GF_TRY(dalloc(&re_w, n0));
   if (p.inner_krylov_steps > 0) GF_TRY(ik.alloc(p.inner_krylov_restart, n0));
   if (p.root_local_correction_mode == 2)
@@ -661,6 +661,7 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
 
 void Gmg::release() {
   GF_DEVICE_LOCK();
+  dfree(d_err);
   for (int l = 0; l < (int)lv.size(); ++l) {
     Level& v = lv[l];
     if (l > 0) dfree(v.d_f2f);
@@ -1022,14 +1023,21 @@ int Gmg::fgmres_dev(Krylov& K, const std::function<int(const double*, double*, c
       double* w = K.V + (long)(j + 1) * n;
       GF_TRY(M(Vj, Zj, step));
       GF_TRY(level_op(0, Zj, w, EPI_PLAIN, EpiArgs<double>{}, step, st));
-      GF_TRY(dot_f64(work, n, K.V, w, K.H + j, false, step, st));
-      for (int i = 0; i < j; ++i)
-        GF_TRY(mgs_axpy_dot(work, n, K.V + (long)i * n, w, K.H + (long)i * m + j,
-                            K.V + (long)(i + 1) * n, K.H + (long)(i + 1) * m + j, false, step,
-                            st));
-      GF_TRY(mgs_axpy_dot(work, n, Vj, w, K.H + (long)j * m + j, nullptr,
-                          K.H + (long)(j + 1) * m + j, true, step, st));
-      GF_TRY(scale_inv(n, w, K.H + (long)(j + 1) * m + j, 1e-14, w, step, st));
+      int swept = 0;
+      if (p.mgs_sweep) {  // plain launch: this sequence is recorded in a graph
+        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, step, 0, st);
+        if (swept < 0) return swept;
+      }
+      if (!swept) {
+        GF_TRY(dot_f64(work, n, K.V, w, K.H + j, false, step, st));
+        for (int i = 0; i < j; ++i)
+          GF_TRY(mgs_axpy_dot(work, n, K.V + (long)i * n, w, K.H + (long)i * m + j,
+                              K.V + (long)(i + 1) * n, K.H + (long)(i + 1) * m + j, false,
+                              step, st));
+        GF_TRY(mgs_axpy_dot(work, n, Vj, w, K.H + (long)j * m + j, nullptr,
+                            K.H + (long)(j + 1) * m + j, true, step, st));
+        GF_TRY(scale_inv(n, w, K.H + (long)(j + 1) * m + j, 1e-14, w, step, st));
+      }
       LAUNCH1(k_givens, st, m, j, tol, K.H, K.cs, K.sn, K.g, K.sc, K.fl);
     }
     hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(256), 0, st, m, K.H, K.g, K.y, K.fl);
@@ -1390,16 +1398,35 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
       double* w = K.V + (long)(j + 1) * n;
       GF_TRY(apply(Vj, Zj, st));
       GF_TRY(level_op(0, Zj, w, EPI_PLAIN, EpiArgs<double>{}, nullptr, st));
-      GF_TRY(dot_f64(work, n, K.V, w, K.H + j, false, nullptr, st));
-      for (int i = 0; i < j; ++i)
-        GF_TRY(mgs_axpy_dot(work, n, K.V + (long)i * n, w, K.H + (long)i * m + j,
-                            K.V + (long)(i + 1) * n, K.H + (long)(i + 1) * m + j, false,
-                            nullptr, st));
-      GF_TRY(mgs_axpy_dot(work, n, Vj, w, K.H + (long)j * m + j, nullptr,
-                          K.H + (long)(j + 1) * m + j, true, nullptr, st));
-      GF_TRY(scale_inv(n, w, K.H + (long)(j + 1) * m + j, 1e-14, w, nullptr, st));
+      int swept = 0;
+      if (p.mgs_sweep) {
+        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, nullptr, p.mgs_sweep == 2, st);
+        if (swept < 0) return swept;
+      }
+      if (!swept) {
+        GF_TRY(dot_f64(work, n, K.V, w, K.H + j, false, nullptr, st));
+        for (int i = 0; i < j; ++i)
+          GF_TRY(mgs_axpy_dot(work, n, K.V + (long)i * n, w, K.H + (long)i * m + j,
+                              K.V + (long)(i + 1) * n, K.H + (long)(i + 1) * m + j, false,
+                              nullptr, st));
+        GF_TRY(mgs_axpy_dot(work, n, Vj, w, K.H + (long)j * m + j, nullptr,
+                            K.H + (long)(j + 1) * m + j, true, nullptr, st));
+        GF_TRY(scale_inv(n, w, K.H + (long)(j + 1) * m + j, 1e-14, w, nullptr, st));
+      }
       LAUNCH1(k_givens, st, m, j, tol, K.H, K.cs, K.sn, K.g, K.sc, K.fl);
-      GF_TRY(read_scalars(K.sc + 2, h, 2, st));
+      // projected residual, h_{j+1,j} and the sweep error flag in one wait
+      GF_HIP(hipMemcpyAsync(hs, K.sc + 2, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+      GF_HIP(hipMemcpyAsync(hs + 2, d_err, sizeof(double), hipMemcpyDeviceToHost, st));
+      GF_HIP(hipStreamSynchronize(st));
+      std::memcpy(h, hs, sizeof(double) * 3);
+      if (h[2] != 0.0) {
+        // flags may now disagree: force a reset before the next sweep
+        work.sweep_G = -1;
+        drop_graph();
+        (void)hipMemsetAsync(d_err, 0, sizeof(double), st);
+        set_error("gf_gmg_solve: a grid-wide wait of the Gram-Schmidt sweep exceeded its bound");
+        return -1;
+      }
       ++total;
       push(h[0]);
       if (h[0] <= tol || h[1] <= 1e-14) {

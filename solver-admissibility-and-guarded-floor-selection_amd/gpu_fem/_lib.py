@@ -47,6 +47,7 @@ class gf_gmg_params(C.Structure):
         ("inner_krylov_restart", C.c_int32),
         ("coarsest_solve", C.c_int32),
         ("graph_apply", C.c_int32),
+        ("mgs_sweep", C.c_int32),
     ]
 
 

@@ -58,7 +58,7 @@ class GalerkinMatFreeGMG:
                  root_local_correction_inner_steps: int = 10,
                  root_local_correction_inner_restart=None, inner_krylov_steps: int = 0,
                  inner_krylov_restart=None, coarsest_solve: str = "inverse",
-                 graph_apply: bool = True) -> None:
+                 graph_apply: bool = True, mgs_sweep=True) -> None:
         # argument validation (multigrid_v4.py:600-699), same messages
         if smoother_type not in {"chebyshev", "jacobi"}:
             raise ValueError(f"smoother_type must be 'chebyshev' or 'jacobi'; got {smoother_type!r}")
@@ -163,6 +163,11 @@ class GalerkinMatFreeGMG:
         p.coarsest_solve = 0 if coarsest_solve == "inverse" else 1
         # record the whole application as a hipGraph after each set-up
         p.graph_apply = 1 if graph_apply else 0
+        # outer FGMRES: one cooperative launch per Arnoldi orthogonalisation
+        # (True/1 persistent launch, 2 cooperative launch, False/0 per projection)
+        if int(mgs_sweep) not in (0, 1, 2):
+            raise ValueError("mgs_sweep must be False/0, True/1 or 2")
+        p.mgs_sweep = int(mgs_sweep)
         self._params = p
         h = C.c_void_p()
         _lib.check(lib.gf_gmg_create(C.byref(h), mf_op.ctx_handle, C.byref(p)), "gf_gmg_create")
