@@ -130,7 +130,7 @@ __device__ __forceinline__ void coarse_epi(int i, T ax, const T* __restrict__ x,
 // (27 slots used); slot products Sum_q A[k][r][q] x[J_k, q] meet in LDS and
 // lane r < 3 of each node adds them in slot order before the epilogue.  The
 // small coarse grids get 27 times the parallelism of a thread-per-node map.
-template <typename T, int EPI>
+template <typename T, int EPI, bool OFF>
 __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T* __restrict__ A,
                                                     const T* __restrict__ x, T* __restrict__ y,
                                                     EpiArgs<T> ep, const int* gate) {
@@ -143,19 +143,18 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
     int ix, iy, iz;
     node_xyz(g, n, ix, iy, iz);
     const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+    const bool in = (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) & (jz < g.NZ);
+    const int m = (jx * g.NY + jy) * g.NZ + jz;
+    const long nn = g.n_nodes;
     T s0 = 0, s1 = 0, s2 = 0;
-    if (jx >= 0 && jx < g.NX && jy >= 0 && jy < g.NY && jz >= 0 && jz < g.NZ) {
-      const int m = (jx * g.NY + jy) * g.NZ + jz;
-      const long nn = g.n_nodes;
+    // masked loads read 0 (branch-free); a zero x contributes exact zeros, as
+    // a skipped term does
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int f = map(3 * m + q);
-        if (f < 0) continue;
-        const T xv = x[f];
-        s0 += A[(k * 9 + 0 * 3 + q) * nn + n] * xv;
-        s1 += A[(k * 9 + 1 * 3 + q) * nn + n] * xv;
-        s2 += A[(k * 9 + 2 * 3 + q) * nn + n] * xv;
-      }
+    for (int q = 0; q < 3; ++q) {
+      const T xv = map_load<OFF>(map, x, 3 * m + q, in);
+      s0 += A[(k * 9 + 0 * 3 + q) * nn + n] * xv;
+      s1 += A[(k * 9 + 1 * 3 + q) * nn + n] * xv;
+      s2 += A[(k * 9 + 2 * 3 + q) * nn + n] * xv;
     }
     part[ln][k][0] = s0;
     part[ln][k][1] = s1;
@@ -163,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
   }
   __syncthreads();
   if (!valid || k >= 3) return;
-  const int o = map(3 * n + k);
+  const int o = map_idx<OFF>(map, 3 * n + k);
   if (o < 0) return;
   T acc = 0;
 #pragma unroll
@@ -171,31 +170,45 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
   coarse_epi<T, EPI>(o, acc, x, y, ep);
 }
 
-template <typename T>
-int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
-                   const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
+template <typename T, bool OFF>
+static void launch_stencil(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
+                           const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
   dim3 grid(cdiv(g.n_nodes, 8)), block(kBlock);
   switch (epi) {
     case EPI_PLAIN:
-      hipLaunchKernelGGL((k_stencil<T, EPI_PLAIN>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+      hipLaunchKernelGGL((k_stencil<T, EPI_PLAIN, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
+                         gate);
       break;
     case EPI_RESID:
-      hipLaunchKernelGGL((k_stencil<T, EPI_RESID>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+      hipLaunchKernelGGL((k_stencil<T, EPI_RESID, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
+                         gate);
       break;
     case EPI_CHEB0:
-      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB0>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB0, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
+                         gate);
       break;
     case EPI_CHEB:
-      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB>), grid, block, 0, st, g, map, A, x, y, ep, gate);
-      break;
-    case EPI_JACOBI:
-      hipLaunchKernelGGL((k_stencil<T, EPI_JACOBI>), grid, block, 0, st, g, map, A, x, y, ep,
+      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
                          gate);
       break;
     default:
-      set_error("stencil_matvec: bad epilogue");
-      return -2;
+      hipLaunchKernelGGL((k_stencil<T, EPI_JACOBI, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
+                         gate);
+      break;
   }
+}
+
+template <typename T>
+int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
+                   const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
+  if (epi < EPI_PLAIN || epi > EPI_JACOBI) {
+    set_error("stencil_matvec: bad epilogue");
+    return -2;
+  }
+  if (map.off >= 0)
+    launch_stencil<T, true>(g, map, A, x, y, epi, ep, gate, st);
+  else
+    launch_stencil<T, false>(g, map, A, x, y, epi, ep, gate, st);
   GF_CHECK_LAUNCH();
   return 0;
 }

@@ -120,6 +120,30 @@ __device__ __forceinline__ void store_ack_wait() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Free index of a full DOF with the map mode fixed at compile time.
+template <bool OFF>
+__device__ __forceinline__ int map_idx(const DofMap& m, int dof) {
+  if (OFF) return dof >= m.off ? dof - m.off : -1;
+  return m.f2f[dof];
+}
+
+// Branch-free read of u at full DOF `dof` through the map: DOFs outside the
+// grid (in == false) or fixed read as 0.  The load always targets a valid
+// address (index 0 when masked), so every load of a window can be issued
+// back to back instead of behind an exec-mask branch.
+template <bool OFF, typename T>
+__device__ __forceinline__ T map_load(const DofMap& m, const T* __restrict__ u, int dof, bool in) {
+  int f;
+  if (OFF) {
+    f = dof - m.off;
+  } else {
+    f = m.f2f[in ? dof : 0];
+  }
+  const bool ok = in & (f >= 0);
+  const T v = u[ok ? f : 0];
+  return ok ? v : T(0);
+}
+
 // Device scalar slots (FP64) used by device-resident control flow.
 struct Scalars {
   double* s;   // base pointer of a device array of doubles

@@ -75,7 +75,7 @@ __device__ __forceinline__ void elem_rows(const T* __restrict__ Ke, const T (&un
 }
 
 // SPLIT = 1: one lane per node, the whole tree in registers.
-template <typename T, int EPI>
+template <typename T, int EPI, bool OFF>
 __global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
                                                              const T* __restrict__ Ke,
                                                              const T* __restrict__ E,
@@ -86,7 +86,8 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
   if (gate && *gate == 0) return;
   const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= g.n_nodes) return;
-  const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
+  const int o0 = map_idx<OFF>(map, 3 * n), o1 = map_idx<OFF>(map, 3 * n + 1),
+            o2 = map_idx<OFF>(map, 3 * n + 2);
   if ((o0 & o1 & o2) < 0) return;
   const int iz = n % g.NZ, t = n / g.NZ, iy = t % g.NY, ix = t / g.NY;
   T un[27][3];
@@ -101,10 +102,8 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
                         (jz < g.NZ);
         const int m = (jx * g.NY + jy) * g.NZ + jz;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const int f = in ? map(3 * m + c) : -1;
-          un[dx * 9 + dy * 3 + dz][c] = f >= 0 ? u[f] : T(0);
-        }
+        for (int c = 0; c < 3; ++c)
+          un[dx * 9 + dy * 3 + dz][c] = map_load<OFF>(map, u, 3 * m + c, in);
       }
   A Q[2][3];
 #pragma unroll
@@ -120,7 +119,8 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
         const int ex = ix - 1 + a, ey = iy - 1 + b, ez = iz - 1 + c;
         const bool in = (ex >= 0) & (ex < g.nx) & (ey >= 0) & (ey < g.ny) & (ez >= 0) &
                         (ez < g.nz);
-        const T e = in ? E[(ex * g.ny + ey) * g.nz + ez] : T(0);
+        const T ev = E[in ? (ex * g.ny + ey) * g.nz + ez : 0];
+        const T e = in ? ev : T(0);
         T t0, t1, t2;
         elem_rows<T>(Ke, un, a, b, c, t0, t1, t2);
         P[b][0] += (A)(e * t0);
@@ -165,7 +165,7 @@ __device__ __forceinline__ void pair_rows(const T* __restrict__ Ke, const T (&w)
   }
 }
 
-template <typename T, int EPI>
+template <typename T, int EPI, bool OFF>
 __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map,
                                                               const T* __restrict__ Ke,
                                                               const T* __restrict__ E,
@@ -195,10 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
                         (jz < g.NZ);
         const int m = (jx * g.NY + jy) * g.NZ + jz;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const int f = in ? map(3 * m + q) : -1;
-          w[px][py][pz][q] = f >= 0 ? u[f] : T(0);
-        }
+        for (int q = 0; q < 3; ++q) w[px][py][pz][q] = map_load<OFF>(map, u, 3 * m + q, in);
       }
   A P[3] = {0, 0, 0};
 #pragma unroll
@@ -206,7 +203,8 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
     const int ex = ix - 1 + a, ey = iy - 1 + b, ez = iz - 1 + c;
     const bool in = valid & (ex >= 0) & (ex < g.nx) & (ey >= 0) & (ey < g.ny) & (ez >= 0) &
                     (ez < g.nz);
-    const T e = in ? E[(ex * g.ny + ey) * g.nz + ez] : T(0);
+    const T ev = E[in ? (ex * g.ny + ey) * g.nz + ez : 0];
+    const T e = in ? ev : T(0);
     T t0, t1, t2;
     pair_rows<T>(Ke, w, a, b, c, t0, t1, t2);
     P[0] += (A)(e * t0);
@@ -219,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
   }
   __syncthreads();
   if (wv != 0 || !valid) return;
-  const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
+  const int o0 = map_idx<OFF>(map, 3 * n), o1 = map_idx<OFF>(map, 3 * n + 1),
+            o2 = map_idx<OFF>(map, 3 * n + 2);
   A r[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q)
@@ -237,15 +236,25 @@ void set_fine_split(int split_f64, int split_f32) {
 }
 int get_fine_split(int is_f64) { return is_f64 ? g_split_f64 : g_split_f32; }
 
+template <typename T, int EPI, bool OFF>
+static void launch_fine_m(int split, const Grid& g, const DofMap& map, const T* Ke, const T* E,
+                          const T* u, T* y, const EpiArgs<T>& ep, const int* gate,
+                          hipStream_t st) {
+  if (split == 1)
+    hipLaunchKernelGGL((k_fine_matvec_node<T, EPI, OFF>), dim3(cdiv(g.n_nodes, kBlock)),
+                       dim3(kBlock), 0, st, g, map, Ke, E, u, y, ep, gate);
+  else
+    hipLaunchKernelGGL((k_fine_matvec_wave4<T, EPI, OFF>), dim3(cdiv(g.n_nodes, 64)),
+                       dim3(kBlock), 0, st, g, map, Ke, E, u, y, ep, gate);
+}
+
 template <typename T, int EPI>
 static void launch_fine(int split, const Grid& g, const DofMap& map, const T* Ke, const T* E,
                         const T* u, T* y, const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
-  if (split == 1)
-    hipLaunchKernelGGL((k_fine_matvec_node<T, EPI>), dim3(cdiv(g.n_nodes, kBlock)), dim3(kBlock),
-                       0, st, g, map, Ke, E, u, y, ep, gate);
+  if (map.off >= 0)
+    launch_fine_m<T, EPI, true>(split, g, map, Ke, E, u, y, ep, gate, st);
   else
-    hipLaunchKernelGGL((k_fine_matvec_wave4<T, EPI>), dim3(cdiv(g.n_nodes, 64)), dim3(kBlock), 0,
-                       st, g, map, Ke, E, u, y, ep, gate);
+    launch_fine_m<T, EPI, false>(split, g, map, Ke, E, u, y, ep, gate, st);
 }
 
 template <typename T>
