@@ -119,12 +119,14 @@ int32_t gf_ctx_create(gf_ctx** out, int32_t nelx, int32_t nely, int32_t nelz,
                       const double* KE, const int32_t* free_dofs, int32_t n_free, int32_t ndof);
 int32_t gf_ctx_destroy(gf_ctx* ctx);
 /* Launch-shape autotuner of the fine operator (no reference counterpart):
- * times the one-lane-per-node and four-waves-per-64-nodes kernels on this
- * grid and keeps the faster for every later call on ctx.  Both shapes give
- * bitwise-identical results.  us_out (nullable, host): 2 floats of timings;
- * shape_out (nullable, host): the chosen shape (1 or 4).  Synchronises. */
+ * times the fine-operator kernels on this grid -- shape 1 one lane per node,
+ * 4 four waves per 64 nodes, 8 and 16 the same with the node and element
+ * windows staged in LDS for an 8x8 / 4x16 (y,z) patch -- and keeps the
+ * fastest for every later call on ctx.  All shapes give bitwise-identical
+ * results.  us_out (nullable, host): 4 floats of timings in that order;
+ * shape_out (nullable, host): the chosen shape.  Synchronises. */
 int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* stream);
-/* Force a launch shape: 0 = process default, 1, or 4. */
+/* Force a launch shape: 0 = process default, 1, 4, 8 or 16. */
 int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape);
 /* y_free = K_ff(E) u_free. E: n_elem moduli; u,y: n_free (device). */
 int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream);

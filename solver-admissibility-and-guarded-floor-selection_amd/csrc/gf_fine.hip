@@ -142,13 +142,13 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
 // and each lane loads only the 2x2x3 node window of its two elements.  The
 // pair sums P_ab meet in LDS and wave 0 forms (P00+P01)+(P10+P11), the same
 // tree as the one-lane-per-node kernel.
-template <typename T>
-__device__ __forceinline__ void pair_rows(const T* __restrict__ Ke, const T (&w)[2][2][3][3],
-                                          int a, int b, int c, T& t0, T& t1, T& t2) {
+template <typename T, typename KP>
+__device__ __forceinline__ void pair_rows(KP Ke, const T (&w)[2][2][3][3], int a, int b, int c,
+                                          T& t0, T& t1, T& t2) {
   const int L = (1 - c) * 4 + ((1 - b) ? ((1 - a) ? 2 : 3) : ((1 - a) ? 1 : 0));
-  const T* K0 = Ke + (3 * L + 0) * 24;
-  const T* K1 = Ke + (3 * L + 1) * 24;
-  const T* K2 = Ke + (3 * L + 2) * 24;
+  const KP K0 = Ke + (3 * L + 0) * 24;
+  const KP K1 = Ke + (3 * L + 1) * 24;
+  const KP K2 = Ke + (3 * L + 2) * 24;
   t0 = 0;
   t1 = 0;
   t2 = 0;
@@ -228,11 +228,103 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
   if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
 }
 
+// LDS-tiled form of the four-wave kernel.  A block covers a TY x TZ patch of
+// nodes in one x-plane (64 nodes) and first stages, with branch-free masked
+// loads, the u values of the 3 x (TY+2) x (TZ+2) node window and the moduli
+// of the 2 x (TY+1) x (TZ+1) element window in LDS (zeros outside the grid
+// and on fixed DOFs).  Wave w then forms the element pair (a,b) = (w>>1, w&1)
+// exactly as k_fine_matvec_wave4 does, reading its window from LDS, and the
+// pair sums meet in the same fixed tree, so the result is bitwise identical.
+// One global read per window entry replaces the 36-per-lane gathers.
+template <typename T, int EPI, bool OFF, int TZ>
+__global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
+                                                             const T* __restrict__ Ke,
+                                                             const T* __restrict__ E,
+                                                             const T* __restrict__ u,
+                                                             T* __restrict__ y, EpiArgs<T> ep,
+                                                             const int* gate) {
+  using A = typename Acc<T>::type;
+  constexpr int TY = 64 / TZ;
+  constexpr int WY = TY + 2, WZ = TZ + 2;      // node window (x: 3 planes)
+  constexpr int EY = TY + 1, EZ = TZ + 1;      // element window (x: 2 planes)
+  constexpr int NU = 3 * WY * WZ * 3, NE = 2 * EY * EZ;
+  if (gate && *gate == 0) return;
+  __shared__ T us[NU];
+  __shared__ T es[NE];
+  __shared__ A part[3][3][64];
+  const int tiles_z = (g.NZ + TZ - 1) / TZ, tiles_y = (g.NY + TY - 1) / TY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, ix = tile / (tiles_z * tiles_y);
+  const int y0 = ty * TY, z0 = tz * TZ;
+  // stage the node window: entry ((xx*WY + yy)*WZ + zz)*3 + q is node
+  // (ix-1+xx, y0-1+yy, z0-1+zz), component q
+  for (int t = threadIdx.x; t < NU; t += kBlock) {
+    const int q = t % 3, r = t / 3;
+    const int zz = r % WZ, r2 = r / WZ, yy = r2 % WY, xx = r2 / WY;
+    const int jx = ix - 1 + xx, jy = y0 - 1 + yy, jz = z0 - 1 + zz;
+    const bool in = (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) & (jz < g.NZ);
+    us[t] = map_load<OFF>(map, u, 3 * ((jx * g.NY + jy) * g.NZ + jz) + q, in);
+  }
+  // element window: entry (ex*EY + ey)*EZ + ez is element (ix-1+ex, y0-1+ey, z0-1+ez)
+  for (int t = threadIdx.x; t < NE; t += kBlock) {
+    const int ez = t % EZ, r = t / EZ, ey = r % EY, ex = r / EY;
+    const int cx = ix - 1 + ex, cy = y0 - 1 + ey, cz = z0 - 1 + ez;
+    const bool in = (cx >= 0) & (cx < g.nx) & (cy >= 0) & (cy < g.ny) & (cz >= 0) & (cz < g.nz);
+    const T ev = E[in ? (cx * g.ny + cy) * g.nz + cz : 0];
+    es[t] = in ? ev : T(0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int a = wv >> 1, b = wv & 1;
+  const int ly = lane / TZ, lz = lane % TZ;
+  const int iy = y0 + ly, iz = z0 + lz;
+  const bool valid = (iy < g.NY) & (iz < g.NZ);
+  T w[2][2][3][3];
+#pragma unroll
+  for (int px = 0; px < 2; ++px)
+#pragma unroll
+    for (int py = 0; py < 2; ++py)
+#pragma unroll
+      for (int pz = 0; pz < 3; ++pz)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          w[px][py][pz][q] = us[(((a + px) * WY + (ly + b + py)) * WZ + (lz + pz)) * 3 + q];
+  A P[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const T e = es[(a * EY + (ly + b)) * EZ + (lz + c)];
+    T t0, t1, t2;
+    pair_rows<T>(Ke, w, a, b, c, t0, t1, t2);
+    P[0] += (A)(e * t0);
+    P[1] += (A)(e * t1);
+    P[2] += (A)(e * t2);
+  }
+  if (wv > 0) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) part[wv - 1][r][lane] = P[r];
+  }
+  __syncthreads();
+  if (wv != 0 || !valid) return;
+  const int n = (ix * g.NY + iy) * g.NZ + iz;
+  const int o0 = map_idx<OFF>(map, 3 * n), o1 = map_idx<OFF>(map, 3 * n + 1),
+            o2 = map_idx<OFF>(map, 3 * n + 2);
+  A r[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    r[q] = (P[q] + part[0][q][lane]) + (part[1][q][lane] + part[2][q][lane]);
+  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)r[0], u, y, ep);
+  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)r[1], u, y, ep);
+  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
+}
+
 static std::atomic<int> g_split_f64{4}, g_split_f32{4};
 
+static bool valid_split(int s) { return s == 1 || s == 4 || s == 8 || s == 16; }
+
 void set_fine_split(int split_f64, int split_f32) {
-  if (split_f64 == 1 || split_f64 == 4) g_split_f64 = split_f64;
-  if (split_f32 == 1 || split_f32 == 4) g_split_f32 = split_f32;
+  if (valid_split(split_f64)) g_split_f64 = split_f64;
+  if (valid_split(split_f32)) g_split_f32 = split_f32;
 }
 int get_fine_split(int is_f64) { return is_f64 ? g_split_f64 : g_split_f32; }
 
@@ -240,12 +332,22 @@ template <typename T, int EPI, bool OFF>
 static void launch_fine_m(int split, const Grid& g, const DofMap& map, const T* Ke, const T* E,
                           const T* u, T* y, const EpiArgs<T>& ep, const int* gate,
                           hipStream_t st) {
-  if (split == 1)
+  if (split == 1) {
     hipLaunchKernelGGL((k_fine_matvec_node<T, EPI, OFF>), dim3(cdiv(g.n_nodes, kBlock)),
                        dim3(kBlock), 0, st, g, map, Ke, E, u, y, ep, gate);
-  else
+  } else if (split == 8 || split == 16) {
+    const int tz = split, ty = 64 / tz;
+    const int tiles = g.NX * cdiv(g.NY, ty) * cdiv(g.NZ, tz);
+    if (split == 8)
+      hipLaunchKernelGGL((k_fine_matvec_tile<T, EPI, OFF, 8>), dim3(tiles), dim3(kBlock), 0, st,
+                         g, map, Ke, E, u, y, ep, gate);
+    else
+      hipLaunchKernelGGL((k_fine_matvec_tile<T, EPI, OFF, 16>), dim3(tiles), dim3(kBlock), 0,
+                         st, g, map, Ke, E, u, y, ep, gate);
+  } else {
     hipLaunchKernelGGL((k_fine_matvec_wave4<T, EPI, OFF>), dim3(cdiv(g.n_nodes, 64)),
                        dim3(kBlock), 0, st, g, map, Ke, E, u, y, ep, gate);
+  }
 }
 
 template <typename T, int EPI>

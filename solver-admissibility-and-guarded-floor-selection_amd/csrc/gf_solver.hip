@@ -359,6 +359,7 @@ int Ctx::nodeblock_alloc() {
 }
 
 int Ctx::autotune(hipStream_t st, float* us_out) {
+  GF_DEVICE_LOCK();
   const long ne = (long)g.nx * g.ny * g.nz;
   double *E = nullptr, *u = nullptr, *y = nullptr;
   GF_TRY(dalloc(&E, ne));
@@ -370,10 +371,11 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
   hipEvent_t e0, e1;
   GF_HIP(hipEventCreate(&e0));
   GF_HIP(hipEventCreate(&e1));
-  const int shapes[2] = {1, 4};
+  const int shapes[4] = {1, 4, 8, 16};
   float best = 1e30f;
   int best_shape = 4;
-  for (int s : shapes) {
+  for (int si = 0; si < 4; ++si) {
+    const int s = shapes[si];
     Grid gg = g;
     gg.shape = s;
     for (int w = 0; w < 3; ++w)
@@ -387,7 +389,7 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
     float ms = 0.f;
     GF_HIP(hipEventElapsedTime(&ms, e0, e1));
     const float us = ms * 1000.f / reps;
-    if (us_out) us_out[s == 1 ? 0 : 1] = us;
+    if (us_out) us_out[si] = us;
     if (us < best) {
       best = us;
       best_shape = s;

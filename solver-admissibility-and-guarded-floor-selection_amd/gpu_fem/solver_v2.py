@@ -109,15 +109,21 @@ class MatrixFreeKff:
             self.autotune()
 
     def autotune(self):
-        """Pick the faster fine-operator launch shape for this grid (results
+        """Pick the fastest fine-operator launch shape for this grid (results
         are bitwise identical for every shape)."""
-        us = (C.c_float * 2)()
+        us = (C.c_float * 4)()
         shape = C.c_int32()
         _lib.call("gf_ctx_autotune", self._ctx.h, C.cast(us, C.c_void_p), C.byref(shape),
                   _lib.stream_handle())
         self.launch_shape = int(shape.value)
-        self.launch_timings_us = {"node": float(us[0]), "wave4": float(us[1])}
+        self.launch_timings_us = {"node": float(us[0]), "wave4": float(us[1]),
+                                  "tile8x8": float(us[2]), "tile4x16": float(us[3])}
         return self.launch_shape
+
+    def set_launch_shape(self, shape: int):
+        """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles)."""
+        _lib.call("gf_ctx_set_shape", self._ctx.h, int(shape))
+        self.launch_shape = int(shape)
 
     @property
     def ctx_handle(self):

@@ -258,21 +258,33 @@ def test_coarsest_inverse_matches_triangular(dims, rho_min):
     assert np.abs(res["inverse"] - res["triangular"]).max() <= 100 * kappa * 2.2e-16 * scale
 
 
-def test_fine_launch_shapes_bitwise_identical():
+@pytest.mark.parametrize("dims,kind", [((21, 11, 9), "bridge"), ((19, 9, 13), "tip"),
+                                       ((12, 30, 6), "tip")])
+def test_fine_launch_shapes_bitwise_identical(dims, kind):
+    """Every fine-operator launch shape (lane per node, four waves per 64
+    nodes, LDS tiles 8x8 and 4x16) gives the same bits, FP64 and FP32, with
+    the fused residual epilogue too (table- and offset-mode DOF maps, partial
+    tiles)."""
     from gpu_fem import _lib
 
-    s = _setup((21, 11, 9), "bridge", rho_min=1e-3)
+    s = _setup(dims, kind, rho_min=1e-3)
     mf_g = s["comps"][6]
-    u = _t(np.random.default_rng(4).standard_normal(len(s["free"])))
+    rng = np.random.default_rng(4)
+    u = _t(rng.standard_normal(len(s["free"])))
+    b = _t(rng.standard_normal(len(s["free"])))
     outs = {}
-    for shape in (1, 4):
+    for shape in (1, 4, 8, 16):
         _lib.call("gf_ctx_set_shape", mf_g.ctx_handle, shape)
         outs[shape] = (_n(mf_g.matvec(u, s["Et"])),
-                       _n(mf_g.matvec(u.float(), s["Et"].float())))
-    assert np.array_equal(outs[1][0], outs[4][0])
-    assert np.array_equal(outs[1][1], outs[4][1])
+                       _n(mf_g.matvec(u.float(), s["Et"].float())),
+                       _n(mf_g.residual(b, u, s["Et"])))
+    for shape in (4, 8, 16):
+        for k in range(3):
+            assert np.array_equal(outs[1][k], outs[shape][k]), (shape, k)
+    y_o = s["mf"].matvec(_n(u), s["E"])
+    assert np.abs(outs[8][0] - y_o).max() <= 1e-13 * np.abs(y_o).max()
     shape = mf_g.autotune()
-    assert shape in (1, 4)
+    assert shape in (1, 4, 8, 16)
 
 
 def test_optimized_density_c64_golden(golden_dir):
