@@ -30,26 +30,29 @@ for p in (ROOT, PKG):
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per FP64 matvec launch on the 80x40x20 grid, from rocprofv3 --pmc
-# FETCH_SIZE (1335.69 KiB) and WRITE_SIZE (1614.59 KiB) per dispatch, each
-# scaled by what the counter reads for a 64 MiB device copy (FETCH 32779.5 KiB
-# -> x1.99927; WRITE 65536 KiB -> x1.0): 2.73 MB read + 1.65 MB written
-# against 2.17 + 1.65 MB algorithmic (profiles/fine_matvec_fp64_80x40x20_hbm_traffic.md)
-TRAFFIC_BYTES = 4387805
+# HBM bytes per FP64 matvec launch on the 80x40x20 grid (LDS-tiled kernel),
+# from rocprofv3 --pmc FETCH_SIZE (1297.4 KiB) and WRITE_SIZE (1615.0 KiB) per
+# dispatch, each scaled by what the counter reads for a 64 MiB device copy
+# (FETCH 32779.5 KiB -> x1.99927; WRITE 65536 KiB -> x1.0): 2.66 MB read +
+# 1.65 MB written against 2.17 + 1.65 MB algorithmic
+# (profiles/fine_matvec_fp64_80x40x20_roofline.md)
+TRAFFIC_BYTES = 4309925
+SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
+                 8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--grid", default="80x40x20")
     ap.add_argument("--penal", type=float, default=3.0)
     ap.add_argument("--load", default="edge", choices=["edge", "tip"])
     ap.add_argument("--probabilities", default="0.2,0.35,0.5")
     ap.add_argument("--seeds", default=",".join(str(s) for s in range(32)))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workers", type=int, default=4,
+    ap.add_argument("--workers", type=int, default=8,
                     help="solver instances running states concurrently (one session)")
     ap.add_argument("--mgs-sweep", type=int, default=1, choices=[0, 1, 2],
                     help="outer FGMRES orthogonalisation: 1 persistent sweep, 2 cooperative "
@@ -301,7 +304,8 @@ def main():
             "parallelism": f"single GPU, {args.workers} concurrent solver instances (one session)",
         },
         "roofline": {
-            "kernel": "k_fine_matvec_wave4<double, EPI_PLAIN> (gf_matvec_f64)",
+            "kernel": (f"{SHAPE_KERNELS.get(runner.mf_op.launch_shape, '?')} FP64, "
+                       "plain epilogue (gf_matvec_f64)"),
             "bound": "mfma",
             "achieved": tfs,
             "peak": FP64_PEAK_TFLOPS,
