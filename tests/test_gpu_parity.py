@@ -428,3 +428,40 @@ def test_mgs_sweep_matches_per_projection_launches(dims, rho_min):
     rel = np.abs(h1 - h2) / np.abs(h2)
     assert np.all(rel[:3] <= 1e-10), rel[:5]
     assert np.all(rel <= 0.2)
+
+
+def test_density_field_driver_c64(golden_dir, tmp_path):
+    """The density-field driver on the reference's own input (C64_MF
+    rho_final, cantilever 80x40x20, penal 4.5, tip load): decisions,
+    iteration count, connectivity and density statistics equal the
+    reference's row; r50 within 1e-4 (the reference's replays spread ~1e-8);
+    the CSV carries the reference's columns."""
+    import csv
+    import os
+
+    from gpu_fem import density_field as dfm
+
+    with open(os.path.join(golden_dir, "optimized_density_C64_MF.csv")) as f:
+        ref = [r for r in csv.DictReader(f) if r["density_path"].endswith("rho_final.npy")][0]
+    rho = np.load(os.path.join(golden_dir, "C64_MF__rho_final.npy"))
+    summ, hist = dfm.run_density_fields([rho], names=["C64_MF"])
+    r = summ[0]
+    for k in ("trigger", "attempted_rho_mins"):
+        assert r[k] == ref[k], k
+    for k in ("solve_iters", "solve_converged", "n_floor_attempts", "solid_count",
+              "n_components", "largest_component", "predicted_raise_floor"):
+        assert r[k] == int(ref[k]), k
+    assert r["recommended_rho_min"] == float(ref["recommended_rho_min"])
+    for k in ("density_min", "density_max", "density_mean", "density_grayness"):
+        assert abs(r[k] - float(ref[k])) <= 1e-12 * max(1.0, abs(float(ref[k]))), k
+    assert abs(r["probe_r50"] - float(ref["probe_r50"])) <= 1e-4 * float(ref["probe_r50"])
+    assert r["solve_final_rel_residual"] <= 1e-6
+    assert [h["iter"] for h in hist if h["phase"] == "probe"] == list(range(18))
+    out = tmp_path / "density_detector_summary.csv"
+    dfm.write_csv(str(out), summ, dfm.DENSITY_SUMMARY_FIELDS)
+    with open(out) as f:
+        header = next(csv.reader(f))
+    # the shipped table predates three columns of the current driver
+    # (fine_smoother, post_solve_fallback_triggered, detector_false_keep);
+    # every column it has appears in the same order
+    assert [k for k in header if k in ref] == list(ref.keys())

@@ -91,3 +91,37 @@ def test_classify_and_connectivity_equal_oracle():
         assert _classify(r50, r100, **thr) == opol.classify(r50, r100, **thr)
     m = opol.bernoulli_mask(5, (20, 10, 10), 0.3)
     assert _connectivity(m) == opol.connectivity(m)
+
+
+def test_filtered_sigmoid_density_matches_oracle():
+    from gpu_fem.density_field import filtered_sigmoid_density
+    from oracle.policy import filtered_sigmoid_density as ofsd
+
+    for seed, dims in ((0, (12, 6, 6)), (3, (20, 10, 5))):
+        a = filtered_sigmoid_density(seed, dims)
+        b = ofsd(seed, dims)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+        assert abs(a.mean() - 0.3) < 1e-9 and a.min() > 0.0 and a.max() < 1.0
+
+
+def test_density_csv_schema_and_history_rows(tmp_path):
+    import csv
+
+    from gpu_fem import density_field as dfm
+
+    row = {k: i for i, k in enumerate(dfm.DENSITY_SUMMARY_FIELDS)}
+    row.update(probe_history=[1.0, 0.5, 0.1], solve_history=[(1e-3, [1.0, 1e-7])],
+               extra_key="ignored")
+    p = tmp_path / "summary.csv"
+    dfm.write_csv(str(p), [row], dfm.DENSITY_SUMMARY_FIELDS)
+    with open(p) as f:
+        r = list(csv.reader(f))
+    assert r[0] == dfm.DENSITY_SUMMARY_FIELDS and len(r) == 2
+    h = dfm.history_rows(row, {"density_name": "d", "density_path": "x.npy"}, 1e-12)
+    assert [x["phase"] for x in h] == ["probe"] * 3 + ["recommended_solve"] * 2
+    assert [x["iter"] for x in h] == [0, 1, 2, 0, 1]
+    assert h[3]["rho_min"] == 1e-3 and h[0]["rho_min"] == 1e-12
+    q = tmp_path / "hist.csv"
+    dfm.write_csv(str(q), h, dfm.DENSITY_HISTORY_FIELDS)
+    with open(q) as f:
+        assert next(csv.reader(f)) == dfm.DENSITY_HISTORY_FIELDS
