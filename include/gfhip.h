@@ -15,6 +15,7 @@
  *   gf_matvec_f32, gf_matvec_full_f32, gf_matvec_full_bf16
  *                       FusedMatvec.matvec / .matvec_full  cuda_fused_matvec.py:343-416
  *   gf_diag_f64         MatrixFreeKff.extract_diagonal    src/gpu_fem/solver_v2.py:1228-1251
+ *   gf_elem_sensitivity_f64  compliance sensitivity of SolverV4 src/gpu_fem/solver_v4.py:258-265
  *   gf_nodeblock_*      NodeBlockJacobiMF.update / apply   src/gpu_fem/solver_v2.py:1381-1451
  *   gf_gmg_create       GalerkinMatFreeGMG.__init__       src/gpu_fem/multigrid_v4.py:546-770
  *   gf_gmg_setup        GalerkinMatFreeGMG.setup          src/gpu_fem/multigrid_v4.py:1119-1341
@@ -147,6 +148,12 @@ int32_t gf_matvec_full_f32(gf_ctx* ctx, const float* E, const float* u_full, flo
 int32_t gf_matvec_full_bf16(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
                             void* stream);
 int32_t gf_diag_f64(gf_ctx* ctx, const double* E, double* diag, void* stream);
+/* Element strain energy and SIMP sensitivity of a free-DOF displacement u
+ * (SolverV4._solve_cupy_matfree, solver_v4.py:258-265):
+ *   ce[e] = u_e^T KE u_e (unit modulus), dc[e] = -penal*scale*rho[e]^(penal-1)*ce[e]
+ * with scale = E0 - Emin; ce or dc may be NULL (rho may be NULL when dc is). */
+int32_t gf_elem_sensitivity_f64(gf_ctx* ctx, const double* u, const double* rho, double penal,
+                                double scale, double* ce, double* dc, void* stream);
 int32_t gf_nodeblock_update(gf_ctx* ctx, const double* E, const double* scalar_diag /*nullable*/,
                             void* stream);
 int32_t gf_nodeblock_apply(gf_ctx* ctx, const double* r, double* z, void* stream);

@@ -13,6 +13,9 @@ Each function cites the reference lines it restates.  Pinning (checks in
 * ``MatrixFreeKff``      assembled ``scipy.sparse`` K restricted to free DOFs.
 * ``extract_diagonal``   diagonal of the assembled K_ff.
 * ``NodeBlockJacobiMF``  3x3 diagonal blocks of the assembled K, inverted.
+* ``element_energy``     energy identity sum_e E_e ce_e = u^T K(E) u against
+                         the assembled K, and a finite-difference check of
+                         d(F.u)/d rho_e = -p E'(rho) ce_e (self-adjoint).
 """
 
 from __future__ import annotations
@@ -375,6 +378,12 @@ class NodeBlockJacobiMF:
         if self._partial_local.size > 0 and self._partial_diag_inv is not None:
             z[self._partial_local] = self._partial_diag_inv * r[self._partial_local].astype(np.float64)
         return z.astype(r.dtype)
+
+
+def element_energy(edof: np.ndarray, KE: np.ndarray, u_full: np.ndarray) -> np.ndarray:
+    """ce_e = u_e^T KE u_e (solver_v4.py:258-262: KUe = Ue @ KE, ce = sum(KUe * Ue))."""
+    Ue = u_full[edof]
+    return ((Ue @ KE) * Ue).sum(axis=1)
 
 
 def build_components(spec: Spec, KE=None):

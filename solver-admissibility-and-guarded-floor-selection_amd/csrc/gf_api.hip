@@ -126,6 +126,22 @@ int32_t gf_matvec_full_bf16(gf_ctx* ctx, const float* E, const float* u_full, fl
   return matvec_full_bf16(ctx->c, E, u_full, y_full, S(stream));
 }
 
+int32_t gf_elem_sensitivity_f64(gf_ctx* ctx, const double* u, const double* rho, double penal,
+                                 double scale, double* ce, double* dc, void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(u);
+  if (!ce && !dc) {
+    set_error("gf_elem_sensitivity_f64: ce and dc are both NULL");
+    return -2;
+  }
+  if (dc && !rho) {
+    set_error("gf_elem_sensitivity_f64: dc needs rho");
+    return -2;
+  }
+  return elem_sensitivity(ctx->c.g, ctx->c.map(), ctx->c.Ke, u, rho, penal, scale, ce, dc,
+                          S(stream));
+}
+
 int32_t gf_diag_f64(gf_ctx* ctx, const double* E, double* diag, void* stream) {
   CHECK_PTR(ctx);
   CHECK_PTR(E);

@@ -318,6 +318,56 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
   if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
 }
 
+// Element strain energy and SIMP sensitivity (solver_v4.py:258-265):
+//   ce_e = u_e^T KE u_e  (unit modulus; KUe_j = sum_i u_i KE_ij in i order,
+//          then ce = sum_j KUe_j u_j in j order),
+//   dc_e = -penal * scale * rho_e^(penal-1) * ce_e,  scale = E0 - Emin.
+// One lane per element; u_e is gathered through the DOF map (fixed DOFs
+// read 0, as the reference's full-length U with zeros at fixed DOFs).
+template <bool OFF>
+__global__ __launch_bounds__(kBlock) void k_elem_sens(Grid g, DofMap map,
+                                                      const double* __restrict__ Ke,
+                                                      const double* __restrict__ u,
+                                                      const double* __restrict__ rho, double penal,
+                                                      double scale, double* __restrict__ ce,
+                                                      double* __restrict__ dc) {
+  const long ne = (long)g.nx * g.ny * g.nz;
+  const long e = (long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int ez = (int)(e % g.nz), ey = (int)((e / g.nz) % g.ny), ex = (int)(e / ((long)g.ny * g.nz));
+  double ue[24];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int node = ((ex + lpos(a, 0)) * g.NY + (ey + lpos(a, 1))) * g.NZ + (ez + lpos(a, 2));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) ue[3 * a + q] = map_load<OFF>(map, u, 3 * node + q, true);
+  }
+  double c = 0.0;
+#pragma unroll
+  for (int j = 0; j < 24; ++j) {
+    double k = 0.0;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) k += ue[i] * Ke[i * 24 + j];
+    c += k * ue[j];
+  }
+  if (ce) ce[e] = c;
+  if (dc) dc[e] = -penal * scale * pow(rho[e], penal - 1.0) * c;
+}
+
+int elem_sensitivity(const Grid& g, const DofMap& map, const double* Ke, const double* u,
+                     const double* rho, double penal, double scale, double* ce, double* dc,
+                     hipStream_t st) {
+  const long ne = (long)g.nx * g.ny * g.nz;
+  if (map.off >= 0)
+    hipLaunchKernelGGL(k_elem_sens<true>, dim3(cdiv(ne, kBlock)), dim3(kBlock), 0, st, g, map, Ke,
+                       u, rho, penal, scale, ce, dc);
+  else
+    hipLaunchKernelGGL(k_elem_sens<false>, dim3(cdiv(ne, kBlock)), dim3(kBlock), 0, st, g, map,
+                       Ke, u, rho, penal, scale, ce, dc);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
 static std::atomic<int> g_split_f64{4}, g_split_f32{4};
 
 static bool valid_split(int s) { return s == 1 || s == 4 || s == 8 || s == 16; }

@@ -89,6 +89,10 @@ int get_fine_split(int is_f64);
 template <typename T>
 int zero_start_step(long n, const T* b, const T* dinv, T a, T* dv, T* x, const int* gate,
                     hipStream_t st);
+// ce_e = u_e^T KE u_e and dc_e = -penal*scale*rho_e^(penal-1)*ce_e (either output nullable)
+int elem_sensitivity(const Grid& g, const DofMap& map, const double* Ke, const double* u,
+                     const double* rho, double penal, double scale, double* ce, double* dc,
+                     hipStream_t st);
 int fine_diag(const Grid& g, const DofMap& map, const double* Ke, const double* E, double* diag,
               hipStream_t st);
 // diag -> 1/where(|d|>1e-12, d, 1) (also fp32 copy when d32 != null)

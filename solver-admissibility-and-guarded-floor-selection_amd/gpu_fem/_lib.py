@@ -87,6 +87,7 @@ SIGNATURES = {
     "gf_residual_f64": [_P, _P, _P, _P, _P, _P],
     "gf_matvec_full_f32": [_P, _P, _P, _P, _P],
     "gf_matvec_full_bf16": [_P, _P, _P, _P, _P],
+    "gf_elem_sensitivity_f64": [_P, _P, _P, C.c_double, C.c_double, _P, _P, _P],
     "gf_diag_f64": [_P, _P, _P, _P],
     "gf_nodeblock_update": [_P, _P, _P, _P],
     "gf_nodeblock_apply": [_P, _P, _P, _P],

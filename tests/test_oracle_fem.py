@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 from scipy.sparse.linalg import eigsh
+import scipy.sparse.linalg as spla
 
 from oracle import fem
 
@@ -137,3 +138,40 @@ def test_assembled_operator_matches_reference_eigs(golden_dir, case):
     lmin = float(eigsh(K, k=1, sigma=0.0, which="LM", return_eigenvectors=False)[0])
     assert abs(lmin - float(row["lambda_min"])) <= 1e-6 * abs(float(row["lambda_min"])) + 1e-18
     assert int(row["n_free"]) == len(free)
+
+
+def test_element_energy_identity_and_sensitivity():
+    """sum_e E_e u_e^T KE u_e = u^T K(E) u for any u (assembled K), and for
+    u = K^-1 F the SIMP sensitivity -p (E0-Emin) rho^(p-1) ce_e equals the
+    finite-difference derivative of the compliance F.u (self-adjoint)."""
+    spec = fem.cantilever_spec(6, 3, 3)
+    bc = fem.generate_bc(spec)
+    edof = fem.edof_hex8(6, 3, 3)
+    rng = np.random.default_rng(8)
+    rho = rng.uniform(0.2, 1.0, edof.shape[0])
+    p, Emin = 3.0, 1e-3
+    E = Emin + (1 - Emin) * rho ** p
+    u = rng.standard_normal(bc.ndof)
+    K = fem.assemble_K(edof, E, fem.KE_UNIT_3D, bc.ndof)
+    ce = fem.element_energy(edof, fem.KE_UNIT_3D, u)
+    assert abs((E * ce).sum() - u @ (K @ u)) <= 1e-12 * abs(u @ (K @ u))
+    free = bc.free_dofs
+
+    def compliance(r):
+        Ee = Emin + (1 - Emin) * r ** p
+        Kf = fem.assemble_K(edof, Ee, fem.KE_UNIT_3D, bc.ndof)[free][:, free].tocsc()
+        uf = spla.spsolve(Kf, bc.F[free])
+        return float(bc.F[free] @ uf), uf
+
+    c0, uf = compliance(rho)
+    full = np.zeros(bc.ndof)
+    full[free] = uf
+    dc = -p * (1 - Emin) * rho ** (p - 1) * fem.element_energy(edof, fem.KE_UNIT_3D, full)
+    for e in (0, 17, 40):
+        h = 1e-6
+        r1 = rho.copy()
+        r1[e] += h
+        r2 = rho.copy()
+        r2[e] -= h
+        fd = (compliance(r1)[0] - compliance(r2)[0]) / (2 * h)
+        assert abs(fd - dc[e]) <= 1e-5 * abs(dc).max(), (e, fd, dc[e])
