@@ -96,3 +96,22 @@ def test_guarded_in_loop_policy():
     ev = s.policy_events[-1]
     assert ev["fallback_used"] == 1 and np.isnan(ev["selected_rho_min"])
     assert ev["attempted_rho_mins"].split(";")[-1] == "1e-02"
+
+
+def test_batched_mixed_sizes_match_single_solves():
+    """BatchedFloorSolver: concurrent states of different grids give the same
+    rows as solving each state alone (deterministic kernels), and every
+    accepted solve passes the recomputed-residual guard."""
+    from gpu_fem.batch import BatchedFloorSolver, sample_batch_states
+
+    states = sample_batch_states(5, seed=3, lo=(10, 5, 5), hi=(20, 10, 9))
+    dims = [s["dims"] for s in states]
+    assert len(set(dims)) > 1 and any(d % 2 for s in dims for d in s)
+    par = BatchedFloorSolver(workers=3, restart=40, maxiter=120, precision="fp32").run(states)
+    seq = BatchedFloorSolver(workers=1, restart=40, maxiter=120, precision="fp32").run(states)
+    assert len(par) == len(seq) == 3 * len(states)
+    for a, b in zip(par, seq):
+        for k in ("dims", "rho_min", "iters", "accepted", "true_rel_residual", "compliance"):
+            assert a[k] == b[k], (k, a[k], b[k])
+        if a["accepted"]:
+            assert a["true_rel_residual"] <= 1e-6
