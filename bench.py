@@ -45,14 +45,14 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=12)
     ap.add_argument("--grid", default="80x40x20")
     ap.add_argument("--penal", type=float, default=3.0)
     ap.add_argument("--load", default="edge", choices=["edge", "tip"])
     ap.add_argument("--probabilities", default="0.2,0.35,0.5")
     ap.add_argument("--seeds", default=",".join(str(s) for s in range(32)))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workers", type=int, default=8,
+    ap.add_argument("--workers", type=int, default=12,
                     help="solver instances running states concurrently (one session)")
     ap.add_argument("--mgs-sweep", type=int, default=1, choices=[0, 1, 2],
                     help="outer FGMRES orthogonalisation: 1 persistent sweep, 2 cooperative "

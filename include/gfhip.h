@@ -129,6 +129,11 @@ int32_t gf_ctx_destroy(gf_ctx* ctx);
 int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* stream);
 /* Force a launch shape: 0 = process default, 1, 4, 8 or 16. */
 int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape);
+/* Process-wide launch form of the coarse-level (27-point block stencil)
+ * operator kernels: 1 = one lane per node, slot groups across four waves,
+ * coalesced coefficient reads (default); 0 = one lane per (node, slot).
+ * Both fold the slot products in the same order: identical results. */
+int32_t gf_set_coarse_form(int32_t form);
 /* y_free = K_ff(E) u_free. E: n_elem moduli; u,y: n_free (device). */
 int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream);
 int32_t gf_matvec_f32(gf_ctx* ctx, const float* E, const float* u, float* y, void* stream);

@@ -66,6 +66,15 @@ int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape) {
   return 0;
 }
 
+int32_t gf_set_coarse_form(int32_t form) {
+  if (form != 0 && form != 1) {
+    set_error("gf_set_coarse_form: form must be 0 or 1");
+    return -2;
+  }
+  set_stencil_form(form);
+  return 0;
+}
+
 int32_t gf_ctx_destroy(gf_ctx* ctx) {
   if (!ctx) return 0;
   ctx->c.release();

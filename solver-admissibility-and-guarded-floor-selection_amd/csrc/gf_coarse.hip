@@ -8,6 +8,8 @@
 // CSR matrices value for value.
 #include "gf_internal.h"
 
+#include <atomic>
+
 namespace gf {
 
 __host__ __device__ constexpr int lidx_c(int px, int py, int pz) {
@@ -170,33 +172,83 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
   coarse_epi<T, EPI>(o, acc, x, y, ep);
 }
 
-template <typename T, bool OFF>
-static void launch_stencil(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
-                           const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
-  dim3 grid(cdiv(g.n_nodes, 8)), block(kBlock);
-  switch (epi) {
-    case EPI_PLAIN:
-      hipLaunchKernelGGL((k_stencil<T, EPI_PLAIN, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
-                         gate);
-      break;
-    case EPI_RESID:
-      hipLaunchKernelGGL((k_stencil<T, EPI_RESID, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
-                         gate);
-      break;
-    case EPI_CHEB0:
-      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB0, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
-                         gate);
-      break;
-    case EPI_CHEB:
-      hipLaunchKernelGGL((k_stencil<T, EPI_CHEB, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
-                         gate);
-      break;
-    default:
-      hipLaunchKernelGGL((k_stencil<T, EPI_JACOBI, OFF>), grid, block, 0, st, g, map, A, x, y, ep,
-                         gate);
-      break;
+// Node-per-lane form of k_stencil with coalesced coefficient reads: a block
+// is 64 consecutive nodes x 4 waves; wave w evaluates the slots k = w, w+4,
+// w+8, ... for every node of the block (lane = node, so each coefficient
+// plane A[(k*9 + r*3 + q) * n_nodes + n] is read contiguously), storing the
+// slot products t_k = sum_q A[k][r][q] x[J_k, q] in LDS.  Wave 0 then folds
+// them in slot order 0..26 exactly as k_stencil does, so both forms give the
+// same bits.
+template <typename T, int EPI, bool OFF>
+__global__ __launch_bounds__(kBlock) void k_stencil_nodes(Grid g, DofMap map,
+                                                          const T* __restrict__ A,
+                                                          const T* __restrict__ x,
+                                                          T* __restrict__ y, EpiArgs<T> ep,
+                                                          const int* gate) {
+  if (gate && *gate == 0) return;
+  __shared__ T tk[27][3][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
+  const bool valid = n < g.n_nodes;
+  const int nn_ = valid ? n : 0;
+  int ix, iy, iz;
+  node_xyz(g, nn_, ix, iy, iz);
+  const long nn = g.n_nodes;
+  for (int k = wv; k < 27; k += 4) {
+    const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+    const bool in = valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
+                    (jz < g.NZ);
+    const int m = (jx * g.NY + jy) * g.NZ + jz;
+    T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T xv = map_load<OFF>(map, x, 3 * m + q, in);
+      s0 += A[(k * 9 + 0 * 3 + q) * nn + nn_] * xv;
+      s1 += A[(k * 9 + 1 * 3 + q) * nn + nn_] * xv;
+      s2 += A[(k * 9 + 2 * 3 + q) * nn + nn_] * xv;
+    }
+    tk[k][0][lane] = s0;
+    tk[k][1][lane] = s1;
+    tk[k][2][lane] = s2;
+  }
+  __syncthreads();
+  if (wv != 0 || !valid) return;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int o = map_idx<OFF>(map, 3 * n + r);
+    if (o < 0) continue;
+    T acc = 0;
+#pragma unroll
+    for (int kk = 0; kk < 27; ++kk) acc += tk[kk][r][lane];
+    coarse_epi<T, EPI>(o, acc, x, y, ep);
   }
 }
+
+template <typename T, bool OFF, bool NODES>
+static void launch_stencil(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
+                           const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
+  const dim3 grid(NODES ? cdiv(g.n_nodes, 64) : cdiv(g.n_nodes, 8)), block(kBlock);
+#define GF_STENCIL_LAUNCH(E)                                                                   \
+  if (NODES)                                                                                   \
+    hipLaunchKernelGGL((k_stencil_nodes<T, E, OFF>), grid, block, 0, st, g, map, A, x, y, ep,  \
+                       gate);                                                                  \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_stencil<T, E, OFF>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+  switch (epi) {
+    case EPI_PLAIN: GF_STENCIL_LAUNCH(EPI_PLAIN) break;
+    case EPI_RESID: GF_STENCIL_LAUNCH(EPI_RESID) break;
+    case EPI_CHEB0: GF_STENCIL_LAUNCH(EPI_CHEB0) break;
+    case EPI_CHEB: GF_STENCIL_LAUNCH(EPI_CHEB) break;
+    default: GF_STENCIL_LAUNCH(EPI_JACOBI) break;
+  }
+#undef GF_STENCIL_LAUNCH
+}
+
+// Coarse-operator launch form: 0 = slot-per-lane (k_stencil), 1 = node-per-lane
+// (k_stencil_nodes, default).  Bitwise-identical results.
+static std::atomic<int> g_stencil_form{1};
+void set_stencil_form(int form) { g_stencil_form = form ? 1 : 0; }
 
 template <typename T>
 int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
@@ -205,10 +257,14 @@ int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* 
     set_error("stencil_matvec: bad epilogue");
     return -2;
   }
-  if (map.off >= 0)
-    launch_stencil<T, true>(g, map, A, x, y, epi, ep, gate, st);
-  else
-    launch_stencil<T, false>(g, map, A, x, y, epi, ep, gate, st);
+  const bool nodes = g_stencil_form.load() != 0;
+  if (map.off >= 0) {
+    if (nodes) launch_stencil<T, true, true>(g, map, A, x, y, epi, ep, gate, st);
+    else launch_stencil<T, true, false>(g, map, A, x, y, epi, ep, gate, st);
+  } else {
+    if (nodes) launch_stencil<T, false, true>(g, map, A, x, y, epi, ep, gate, st);
+    else launch_stencil<T, false, false>(g, map, A, x, y, epi, ep, gate, st);
+  }
   GF_CHECK_LAUNCH();
   return 0;
 }

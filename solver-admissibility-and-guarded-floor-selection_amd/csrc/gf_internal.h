@@ -108,6 +108,8 @@ int level1_assemble(const Grid& gc, const DofMap& mapc, const Grid& gf, const do
 template <typename T>
 int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
                    const EpiArgs<T>& ep, const int* gate, hipStream_t st);
+// 0 = slot-per-lane, 1 = node-per-lane coarse operator kernel (same bits)
+void set_stencil_form(int form);
 int stencil_diag(const Grid& g, const DofMap& map, const double* A, double* diag, hipStream_t st);
 int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
             const double* ec, double* yf, const int* gate, hipStream_t st);

@@ -465,3 +465,26 @@ def test_density_field_driver_c64(golden_dir, tmp_path):
     # (fine_smoother, post_solve_fallback_triggered, detector_false_keep);
     # every column it has appears in the same order
     assert [k for k in header if k in ref] == list(ref.keys())
+
+
+def test_coarse_stencil_forms_bitwise_identical():
+    """Node-per-lane and slot-per-lane coarse operator kernels give the same
+    bits on every level (all epilogues run inside a V-cycle)."""
+    from gpu_fem import _lib
+
+    s = _setup((21, 11, 9), "bridge", rho_min=1e-3)
+    gg = s["comps"][7]
+    gg.setup(s["Et"])
+    r = _t(np.random.default_rng(9).standard_normal(len(s["free"])))
+    outs = []
+    try:
+        for form in (0, 1):
+            _lib.call("gf_set_coarse_form", form)
+            lv = [_n(gg.level_matvec(l, _t(np.random.default_rng(l).standard_normal(
+                len(_n(gg.level_dinv(l))))))) for l in range(1, gg._n_levels)]
+            outs.append((_n(gg.vcycle(r)), lv))
+    finally:
+        _lib.call("gf_set_coarse_form", 1)
+    assert np.array_equal(outs[0][0], outs[1][0])
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert np.array_equal(a, b)
