@@ -25,6 +25,9 @@
  *   gf_gmg_solve        _cupy_fgmres(A=MatrixFreeKff.matvec(., E), M=gmg.apply)
  *                                                         src/gpu_fem/multigrid_v4.py:416-534
  *   gf_dot_f64 / gf_nrm2_f64  cp.dot / cp.linalg.norm used by _cupy_fgmres / _cupy_pcg
+ *   gf_cnn_*            learned floor predictor: no reference counterpart (the
+ *                       reference's policy always probes, run_gmg_floor_detector_
+ *                       prospective.py:263-290); its proposals feed the same guard
  */
 #ifndef GFHIP_H
 #define GFHIP_H
@@ -202,6 +205,36 @@ int32_t gf_gmg_level_free(gf_gmg* g, int32_t level, int32_t* out_host);
 int32_t gf_gmg_solve(gf_gmg* g, const double* b, double* x, int32_t x0_given, double tol,
                      int32_t maxiter, int32_t restart, double* history, int32_t hist_cap,
                      int32_t* n_hist, int32_t* iters, int32_t* converged, void* stream);
+
+/* ---- learned floor predictor (no reference counterpart: the reference's
+ * policy always runs the probe; the learned form proposes the floor and r50
+ * from the density and the recomputed-residual guard judges the proposal).
+ * A 3D CNN on the density average-pooled to 16x8x8 plus three grid
+ * features (log2(dim) - 5): conv 3^3 1->16 ReLU, conv 3^3 16->32 ReLU,
+ * global average pool, dense 35->32 ReLU, dense 32->4 (output 0 regresses
+ * the normalised log10 r50, outputs 1..3 are floor-class logits).  BF16
+ * operands with FP32 accumulation in every product, FP32 elsewhere.
+ * theta: n_params floats [W1 16x27 | b1 16 | W2 32x27x16 | b2 32 |
+ * W3 32x35 | b3 32 | W4 4x32 | b4 4].  work: batch * work_floats_per_sample
+ * floats of device scratch owned by the caller.  All pointers are device
+ * pointers; no host syncs; deterministic. */
+int32_t gf_cnn_sizes(int32_t* n_params, int64_t* work_floats_per_sample);
+/* x_out[16*8*8] = adaptive average pool of rho (element order
+ * e = (ix*ny + iy)*nz + iz) over bins [floor(i n/m), ceil((i+1) n/m)). */
+int32_t gf_cnn_pool_f64(const double* rho, int32_t nx, int32_t ny, int32_t nz, float* x_out,
+                        void* stream);
+/* out[batch][4] for x[batch][1024], feat[batch][3]. */
+int32_t gf_cnn_forward(const float* theta, const float* x, const float* feat, int32_t batch,
+                       float* out, float* work, void* stream);
+/* Mean loss (w_reg (out0 - y_reg)^2 + cross-entropy(out1..3, y_cls)) into
+ * loss_out[1] (nullable) and its gradient into grad_out[n_params]; out
+ * (nullable) receives the forward outputs. */
+int32_t gf_cnn_grad(const float* theta, const float* x, const float* feat, const float* y_reg,
+                    const int32_t* y_cls, int32_t batch, float w_reg, float* loss_out,
+                    float* grad_out, float* out, float* work, void* stream);
+/* One Adam step (step counts from 1) on n parameters, FP32. */
+int32_t gf_cnn_adam(float* theta, float* m, float* v, const float* grad, int32_t n, float lr,
+                    float beta1, float beta2, float eps, int32_t step, void* stream);
 
 #ifdef __cplusplus
 }

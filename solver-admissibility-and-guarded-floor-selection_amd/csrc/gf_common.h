@@ -49,6 +49,14 @@ constexpr int kRedBlocks = 512;  // fixed reduction grid => deterministic sums
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__host__ __device__ inline unsigned short bf16_bits_rne(float f) {
+  unsigned int b;
+  __builtin_memcpy(&b, &f, 4);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (unsigned short)(b >> 16);
+}
+
 // Geometry of one level's node grid.
 struct Grid {
   int nx, ny, nz;      // elements

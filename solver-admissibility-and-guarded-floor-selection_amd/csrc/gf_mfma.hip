@@ -24,14 +24,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16 bits, round to nearest even (finite inputs)
-__host__ __device__ inline unsigned short bf16_bits_rne(float f) {
-  unsigned int b;
-  __builtin_memcpy(&b, &f, 4);
-  b += 0x7FFFu + ((b >> 16) & 1u);
-  return (unsigned short)(b >> 16);
-}
-
 __global__ __launch_bounds__(256) void k_elem_bf16(Grid g, const unsigned short* __restrict__ KEb,
                                                    const float* __restrict__ E,
                                                    const float* __restrict__ u,

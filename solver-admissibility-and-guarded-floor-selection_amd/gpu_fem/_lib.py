@@ -110,6 +110,11 @@ SIGNATURES = {
     "gf_gmg_level_free": [_P, _I, _P],
     "gf_gmg_solve": [_P, _P, _P, _I, _D, _I, _I, _P, _I, C.POINTER(_I), C.POINTER(_I),
                      C.POINTER(_I), _P],
+    "gf_cnn_sizes": [C.POINTER(_I), C.POINTER(C.c_int64)],
+    "gf_cnn_pool_f64": [_P, _I, _I, _I, _P, _P],
+    "gf_cnn_forward": [_P, _P, _P, _I, _P, _P, _P],
+    "gf_cnn_grad": [_P, _P, _P, _P, _P, _I, C.c_float, _P, _P, _P, _P, _P],
+    "gf_cnn_adam": [_P, _P, _P, _P, _I, C.c_float, C.c_float, C.c_float, C.c_float, _I, _P],
 }
 _VOID = {"gf_gmg_default_params"}
 

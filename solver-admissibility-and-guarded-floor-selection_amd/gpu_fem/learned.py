@@ -1,0 +1,437 @@
+"""Learned-floor form: a small 3D CNN proposes the floor (and r50) from the
+density, so that the probe solve is skipped; the recomputed-residual guard
+judges every proposal and a rejected one falls back up the plain ladder.
+
+Network (kernels in csrc/gf_cnn.hip, C-ABI gf_cnn_*): density average-pooled
+to 16x8x8 plus grid features log2(dim) - 5 -> conv 3^3 1->16 ReLU -> conv 3^3
+16->32 ReLU -> global average pool -> dense 35->32 ReLU -> dense 32->4
+(output 0: normalised log10 r50, outputs 1..3: logits over the floor ladder
+FLOORS).  Trained on the GPU in BF16 (FP32 accumulation) with Adam, on
+labels that the plain probe/ladder/guard policy (policy.py, the reference's
+run_gmg_floor_detector_prospective.py:191-354 logic) produces for synthetic
+Bernoulli and filtered-sigmoid density states.
+
+The reference has no learned predictor; what is kept from it is the guard
+(``||F - K(E) x|| / ||F|| <= tol`` is the only acceptance test) and the
+ladder order.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .density_field import filtered_sigmoid_density
+from .policy import GuardedFloorPolicy, PolicySession, _sync
+from .presets import cantilever
+
+FLOORS = (1e-12, 1e-3, 1e-2)
+POOL = (16, 8, 8)
+
+
+R50_LOG10_RANGE = (-12.0, 0.5)
+
+
+def r50_target(r50) -> np.ndarray:
+    """Normalised regression target (log10 r50 + 5) / 3, with log10 r50
+    clipped to R50_LOG10_RANGE: a probe that breaks down records r50 = 0 and
+    one that diverges records inf, and neither may dominate the loss."""
+    r = np.asarray(r50, dtype=np.float64)
+    lg = np.log10(np.clip(np.nan_to_num(r, nan=10.0, posinf=10.0), 1e-300, None))
+    return (np.clip(lg, *R50_LOG10_RANGE) + 5.0) / 3.0
+
+
+def r50_from_target(t) -> np.ndarray:
+    return 10.0 ** (3.0 * np.asarray(t, dtype=np.float64) - 5.0)
+
+
+def grid_features(dims) -> np.ndarray:
+    return (np.log2(np.asarray(dims, dtype=np.float64)) - 5.0).astype(np.float32)
+
+
+def floor_class(rho_min: float, floors=FLOORS) -> int:
+    """Index of rho_min in the ladder (nearest in log scale)."""
+    return int(np.argmin([abs(math.log10(rho_min) - math.log10(f)) for f in floors]))
+
+
+def learned_ladder(predicted_class: int, floors, attempt):
+    """Guarded attempts starting at the proposed floor and continuing up the
+    ladder while the guard rejects.  ``attempt(rho_min) -> bool`` (guard
+    verdict).  Returns (attempted floors, accepted index or None,
+    fallback = the proposal itself was rejected)."""
+    if not 0 <= predicted_class < len(floors):
+        raise ValueError("predicted class outside the floor ladder")
+    attempted = []
+    for k in range(predicted_class, len(floors)):
+        attempted.append(floors[k])
+        if attempt(floors[k]):
+            return attempted, k, k != predicted_class
+    return attempted, None, True
+
+
+# --------------------------------------------------------------------------
+# network
+def _sizes():
+    n = C.c_int32()
+    w = C.c_int64()
+    _lib.call("gf_cnn_sizes", C.byref(n), C.byref(w))
+    return int(n.value), int(w.value)
+
+
+def init_theta(seed: int) -> np.ndarray:
+    """He-uniform weights (bound sqrt(6 / fan_in)) in parameter order, zero
+    biases, float32."""
+    rng = np.random.default_rng(seed)
+    blocks = [((16, 27), 27), ((16,), 0), ((32, 27, 16), 432), ((32,), 0), ((32, 35), 35),
+              ((32,), 0), ((4, 32), 32), ((4,), 0)]
+    out = []
+    for shape, fan_in in blocks:
+        if fan_in:
+            b = math.sqrt(6.0 / fan_in)
+            out.append(rng.uniform(-b, b, size=shape).reshape(-1))
+        else:
+            out.append(np.zeros(int(np.prod(shape))))
+    return np.concatenate(out).astype(np.float32)
+
+
+class FloorCNN:
+    """Parameters, Adam state and device scratch of the floor predictor."""
+
+    def __init__(self, theta=None, seed: int = 0):
+        import torch
+
+        _lib.require_gpu()
+        self.n_params, self._wps = _sizes()
+        th = init_theta(seed) if theta is None else np.asarray(theta, dtype=np.float32)
+        if th.shape != (self.n_params,):
+            raise ValueError(f"theta must have {self.n_params} entries")
+        self.theta = torch.as_tensor(th.copy(), device="cuda")
+        self.m = torch.zeros_like(self.theta)
+        self.v = torch.zeros_like(self.theta)
+        self.grad_buf = torch.zeros_like(self.theta)
+        self.loss_buf = torch.zeros(1, dtype=torch.float32, device="cuda")
+        self.step = 0
+        self._work = None
+
+    def _work_for(self, B: int):
+        import torch
+
+        need = B * self._wps
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(need, dtype=torch.float32, device="cuda")
+        return self._work
+
+    @staticmethod
+    def _check(X, feat):
+        if X.dtype.itemsize != 4 or feat.dtype.itemsize != 4:
+            raise ValueError("X and feat must be float32")
+        B = X.shape[0]
+        if X.numel() != B * 1024 or feat.numel() != B * 3:
+            raise ValueError("X must be [B, 16, 8, 8] and feat [B, 3]")
+        return B
+
+    def forward(self, X, feat):
+        """Outputs [B, 4] for pooled densities X [B, 16, 8, 8] and grid
+        features feat [B, 3] (float32 device tensors)."""
+        import torch
+
+        B = self._check(X, feat)
+        out = torch.empty((B, 4), dtype=torch.float32, device="cuda")
+        _lib.call("gf_cnn_forward", _lib.ptr(self.theta), _lib.ptr(X.contiguous()),
+                  _lib.ptr(feat.contiguous()), B, _lib.ptr(out), _lib.ptr(self._work_for(B)),
+                  _lib.stream_handle())
+        return out
+
+    def grad(self, X, feat, y_reg, y_cls, w_reg: float = 1.0):
+        """(mean loss [1], gradient [n_params], outputs [B, 4]) -- device
+        tensors; the gradient buffer is reused by the next call."""
+        import torch
+
+        B = self._check(X, feat)
+        out = torch.empty((B, 4), dtype=torch.float32, device="cuda")
+        _lib.call("gf_cnn_grad", _lib.ptr(self.theta), _lib.ptr(X.contiguous()),
+                  _lib.ptr(feat.contiguous()), _lib.ptr(y_reg.contiguous()),
+                  _lib.ptr(y_cls.contiguous()), B, float(w_reg), _lib.ptr(self.loss_buf),
+                  _lib.ptr(self.grad_buf), _lib.ptr(out), _lib.ptr(self._work_for(B)),
+                  _lib.stream_handle())
+        return self.loss_buf, self.grad_buf, out
+
+    def adam(self, grad, lr: float, beta1: float = 0.9, beta2: float = 0.999,
+             eps: float = 1e-8):
+        self.step += 1
+        _lib.call("gf_cnn_adam", _lib.ptr(self.theta), _lib.ptr(self.m), _lib.ptr(self.v),
+                  _lib.ptr(grad), self.n_params, float(lr), float(beta1), float(beta2),
+                  float(eps), self.step, _lib.stream_handle())
+
+    def fit(self, X, feat, y_reg, y_cls, *, epochs: int = 100, batch: int = 64,
+            lr: float = 3e-3, w_reg: float = 1.0, seed: int = 0, lr_final: float = 3e-4):
+        """Minibatch Adam over the whole set (device tensors); per-epoch mean
+        loss (host floats).  The learning rate decays geometrically from lr
+        to lr_final.  Deterministic for a given seed."""
+        import torch
+
+        n = X.shape[0]
+        rng = np.random.default_rng(seed)
+        hist = []
+        for ep in range(epochs):
+            rate = lr * (lr_final / lr) ** (ep / max(1, epochs - 1))
+            perm = torch.as_tensor(rng.permutation(n), device="cuda")
+            acc = torch.zeros(1, dtype=torch.float32, device="cuda")
+            nb = 0
+            for s in range(0, n, batch):
+                idx = perm[s:s + batch]
+                if idx.numel() < 2:
+                    continue
+                loss, g, _ = self.grad(X.index_select(0, idx), feat.index_select(0, idx),
+                                       y_reg.index_select(0, idx), y_cls.index_select(0, idx),
+                                       w_reg)
+                acc += loss
+                nb += 1
+                self.adam(g, rate)
+            hist.append(float(acc.item()) / max(1, nb))
+        return hist
+
+    def predict(self, X, feat):
+        """(floor class [B] int, predicted r50 [B]) on the host."""
+        out = self.forward(X, feat).double().cpu().numpy()
+        return out[:, 1:].argmax(axis=1), r50_from_target(out[:, 0])
+
+    def theta_host(self) -> np.ndarray:
+        return self.theta.cpu().numpy().copy()
+
+    def save(self, path: str):
+        np.save(path, self.theta_host())
+
+    @classmethod
+    def load(cls, path: str):
+        return cls(theta=np.load(path, allow_pickle=False))
+
+
+def pool_density(rho, dims):
+    """Device float32 [16, 8, 8] pool of a density (host array or device
+    float64 tensor, element order)."""
+    import torch
+
+    r = torch.as_tensor(np.asarray(rho, dtype=np.float64) if not torch.is_tensor(rho) else rho,
+                        dtype=torch.float64, device="cuda").contiguous()
+    nx, ny, nz = (int(v) for v in dims)
+    if r.numel() != nx * ny * nz:
+        raise ValueError("density size does not match the grid")
+    X = torch.empty(POOL, dtype=torch.float32, device="cuda")
+    _lib.call("gf_cnn_pool_f64", _lib.ptr(r), nx, ny, nz, _lib.ptr(X), _lib.stream_handle())
+    return X
+
+
+# --------------------------------------------------------------------------
+# synthetic states
+def sample_states(n: int, seed: int, grids, *, kinds=("bernoulli", "sigmoid"),
+                  p_range=(0.1, 0.5), volfrac_range=(0.2, 0.5)):
+    """Seeded state descriptors: grid drawn from ``grids``, density kind
+    cycling over ``kinds``; Bernoulli solid probability or sigmoid volume
+    fraction uniform in its range; density seed = 100000 * seed + index."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        dims = tuple(int(v) for v in grids[int(rng.integers(len(grids)))])
+        kind = kinds[i % len(kinds)]
+        st = {"dims": dims, "kind": kind, "seed": 100000 * seed + i}
+        if kind == "bernoulli":
+            st["probability"] = float(rng.uniform(*p_range))
+        else:
+            st["volfrac"] = float(rng.uniform(*volfrac_range))
+        out.append(st)
+    return out
+
+
+def density_of(state) -> np.ndarray:
+    """Element densities in [0, 1] of a state descriptor (flat)."""
+    dims = state["dims"]
+    if state["kind"] == "bernoulli":
+        n_e = dims[0] * dims[1] * dims[2]
+        return (np.random.default_rng(state["seed"]).random(n_e)
+                < state["probability"]).astype(np.float64)
+    return filtered_sigmoid_density(state["seed"], dims, volfrac=state["volfrac"])
+
+
+def _policy_state(state, rho):
+    if state["kind"] == "bernoulli":
+        return {"solid": rho > 0.5, "seed": state["seed"],
+                "probability": state.get("probability")}
+    return {"rho_field": rho, "seed": state["seed"], "probability": None}
+
+
+def _groups(states):
+    order = {}
+    for i, s in enumerate(states):
+        order.setdefault(tuple(s["dims"]), []).append(i)
+    return order
+
+
+POLICY_DEFAULTS = dict(penal=3.0, raised_rho_mins=FLOORS[1:], baseline_rho_min=FLOORS[0],
+                       probe_iters=100, probe_r50=50, restart=300, maxiter=300, tol=1e-6)
+
+
+def label_states(states, *, workers: int = 12, load: str = "edge", densities=None, **kw):
+    """Plain guarded policy (probe -> classify -> ladder -> guard) on every
+    state; one session per grid size.  Returns the policy rows in state
+    order, each with ``dims``, ``kind`` and the pooled input ``X`` (host
+    float32 [16, 8, 8]) added."""
+    import torch
+
+    pk = dict(POLICY_DEFAULTS)
+    pk.update(kw)
+    rows = [None] * len(states)
+    for dims, idx in _groups(states).items():
+        session = PolicySession(cantilever(*dims, load=load), workers=min(workers, len(idx)),
+                                **pk)
+        dens = [densities[i] if densities is not None else density_of(states[i]) for i in idx]
+        got = session.run([_policy_state(states[i], d) for i, d in zip(idx, dens)])
+        for i, d, r in zip(idx, dens, got):
+            r.update(dims=dims, kind=states[i]["kind"],
+                     X=pool_density(d, dims).cpu().numpy())
+            r.pop("x", None)
+            rows[i] = r
+        del session
+        torch.cuda.empty_cache()
+    return rows
+
+
+def training_tensors(rows):
+    """(X, feat, y_reg, y_cls) device tensors from labelled rows."""
+    import torch
+
+    X = np.stack([r["X"] for r in rows]).astype(np.float32)
+    feat = np.stack([grid_features(r["dims"]) for r in rows])
+    y_reg = r50_target([r["probe_r50"] for r in rows]).astype(np.float32)
+    y_cls = np.array([floor_class(r["recommended_rho_min"]) for r in rows], dtype=np.int32)
+    dev = lambda a: torch.as_tensor(a, device="cuda")  # noqa: E731
+    return dev(X), dev(feat), dev(y_reg), dev(y_cls)
+
+
+# --------------------------------------------------------------------------
+# the learned policy
+class LearnedFloorPolicy(GuardedFloorPolicy):
+    """Solves a state at the network's proposed floor; the recomputed
+    residual guard accepts it or the ladder continues upward (the plain
+    ladder's raised floors).  Same components and session semantics as
+    GuardedFloorPolicy."""
+
+    def __init__(self, preset_or_spec, *, floors=FLOORS, **kw):
+        kw.setdefault("baseline_rho_min", floors[0])
+        kw.setdefault("raised_rho_mins", tuple(floors[1:]))
+        super().__init__(preset_or_spec, **kw)
+        self.floors = tuple(float(f) for f in floors)
+
+    def first_moduli(self, state: dict) -> np.ndarray:
+        k = state.get("extra", {}).get("predicted_class", 0)
+        return self.moduli(self.floors[k], state.get("solid"), state.get("rho_field"))
+
+    def run_state(self, solid=None, *, rho_field=None, seed=None, probability=None,
+                  keep_x=False, predicted_class=0, predicted_r50=float("nan")):
+        from .solver_v2 import dot
+
+        rec = {"x": None, "final": float("nan"), "iters": 0, "reported": False, "sel": None,
+               "hist": [], "total_iters": 0, "time": 0.0}
+
+        def attempt(rho_min):
+            h: list = []
+            _, x, it, rep, ok, fin, m = self.solve_at_floor(
+                rho_min, maxiter=self.maxiter, history=h, solid=solid, rho_field=rho_field)
+            rec.update(x=x, final=fin, iters=it, reported=rep, sel=m)
+            rec["hist"].append((rho_min, h))
+            rec["total_iters"] += len(h) - 1
+            rec["time"] += m["setup_time_s"] + m["solve_time_s"]
+            return ok
+
+        attempted, acc, fallback = learned_ladder(int(predicted_class), self.floors, attempt)
+        compliance = dot(self.F, rec["x"]) if rec["x"] is not None else float("nan")
+        row = {
+            "preset": self.preset, "seed": seed, "solid_probability": probability,
+            "predicted_class": int(predicted_class),
+            "predicted_rho_min": self.floors[int(predicted_class)],
+            "predicted_r50": float(predicted_r50),
+            "recommended_rho_min": attempted[-1],
+            "attempted_rho_mins": ";".join(f"{v:.0e}" for v in attempted),
+            "n_floor_attempts": len(attempted), "guard_fallback": int(fallback),
+            "solve_converged": int(acc is not None),
+            "solver_reported_converged": int(rec["reported"]),
+            "solve_iters": rec["iters"], "total_iters": rec["total_iters"],
+            "solve_final_rel_residual": rec["final"], "compliance": compliance,
+            "recorded_policy_time_s": rec["time"], "penal": self.penal, "tol": self.tol,
+            "restart": self.restart, "maxiter": self.maxiter,
+            "fine_smoother": self.stack.get("fine_smoother"),
+            "solve_history": rec["hist"],
+        }
+        if keep_x:
+            row["x"] = rec["x"]
+        return row
+
+
+def predict_states(model: FloorCNN, states, densities=None):
+    """Pools every state's density on the device and runs one batched
+    forward; (classes, r50s)."""
+    import torch
+
+    Xs, feats = [], []
+    for i, s in enumerate(states):
+        d = densities[i] if densities is not None else density_of(s)
+        Xs.append(pool_density(d, s["dims"]))
+        feats.append(torch.as_tensor(grid_features(s["dims"]), device="cuda"))
+    return model.predict(torch.stack(Xs), torch.stack(feats))
+
+
+def run_learned(model: FloorCNN, states, *, workers: int = 12, load: str = "edge",
+                densities=None, **kw):
+    """Learned policy on every state (one session per grid size, concurrent
+    instances).  Returns (rows in state order, wall seconds including the
+    pooling and network inference)."""
+    import torch
+
+    pk = dict(POLICY_DEFAULTS)
+    pk.update(kw)
+    dens = densities if densities is not None else [density_of(s) for s in states]
+    _sync()
+    t0 = time.perf_counter()
+    cls, r50 = predict_states(model, states, dens)
+    rows = [None] * len(states)
+    for dims, idx in _groups(states).items():
+        session = PolicySession(cantilever(*dims, load=load), workers=min(workers, len(idx)),
+                                runner_cls=LearnedFloorPolicy, **pk)
+        sts = []
+        for i in idx:
+            st = _policy_state(states[i], dens[i])
+            st["extra"] = {"predicted_class": int(cls[i]), "predicted_r50": float(r50[i])}
+            sts.append(st)
+        for i, r in zip(idx, session.run(sts)):
+            r.update(dims=dims, kind=states[i]["kind"])
+            rows[i] = r
+        del session
+    torch.cuda.synchronize()
+    return rows, time.perf_counter() - t0
+
+
+def compare(plain_rows, learned_rows) -> dict:
+    """Agreement of the accepted floor with the plain ladder, fallback rate,
+    acceptance and worst accepted true residual."""
+    n = len(plain_rows)
+    agree = sum(abs(math.log10(p["recommended_rho_min"]) - math.log10(q["recommended_rho_min"]))
+                < 1e-9 for p, q in zip(plain_rows, learned_rows))
+    acc = [q for q in learned_rows if q["solve_converged"]]
+    return {
+        "states": n,
+        "floor_agreement": agree / max(1, n),
+        "proposal_agreement": sum(floor_class(p["recommended_rho_min"]) == q["predicted_class"]
+                                  for p, q in zip(plain_rows, learned_rows)) / max(1, n),
+        "fallback_rate": sum(q["guard_fallback"] for q in learned_rows) / max(1, n),
+        "learned_accepted": len(acc),
+        "plain_accepted": sum(int(p["solve_converged"]) for p in plain_rows),
+        "max_accepted_true_rel_residual": max([q["solve_final_rel_residual"] for q in acc],
+                                              default=float("nan")),
+        "r50_log10_rmse": float(np.sqrt(np.mean([
+            (math.log10(max(q["predicted_r50"], 1e-300)) - math.log10(max(p["probe_r50"], 1e-300)))
+            ** 2 for p, q in zip(plain_rows, learned_rows)]))),
+    }

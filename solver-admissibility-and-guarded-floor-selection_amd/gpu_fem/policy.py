@@ -164,6 +164,12 @@ class GuardedFloorPolicy:
         return cls(spec, preset_name=f"cantilever_{dims[0]}x{dims[1]}x{dims[2]}_{load}", **kw)
 
     # -------------------------------------------------------------------
+    def first_moduli(self, state: dict) -> np.ndarray:
+        """Moduli of this policy's first set-up on ``state`` (the probe at the
+        baseline floor); a session primes its cached spectral estimate with
+        them."""
+        return self.moduli(self.baseline_rho_min, state.get("solid"), state.get("rho_field"))
+
     def moduli(self, rho_min: float, solid=None, rho_field=None) -> np.ndarray:
         """E = rho_min + (1 - rho_min) rho^p on the host: rho = 1 on the solid
         mask and rho_min elsewhere (prospective:216-222), or a continuous
@@ -308,12 +314,13 @@ class PolicySession:
     single sequential session would produce, whatever the interleaving.
     """
 
-    def __init__(self, preset_or_spec, *, workers: int = 4, **kw):
+    def __init__(self, preset_or_spec, *, workers: int = 4, runner_cls=None, **kw):
         import torch
 
         if workers < 1:
             raise ValueError("workers must be >= 1")
-        self.runners = [GuardedFloorPolicy(preset_or_spec, **kw) for _ in range(workers)]
+        cls = runner_cls or GuardedFloorPolicy
+        self.runners = [cls(preset_or_spec, **kw) for _ in range(workers)]
         self.streams = [torch.cuda.Stream() for _ in range(workers)]
         self._started = False
 
@@ -328,13 +335,15 @@ class PolicySession:
             row = self.runners[k].run_state(st.get("solid"), rho_field=st.get("rho_field"),
                                             seed=st.get("seed"),
                                             probability=st.get("probability"),
-                                            keep_x=st.get("keep_x", False))
+                                            keep_x=st.get("keep_x", False),
+                                            **st.get("extra", {}))
             _sync()
         return row
 
     def run(self, states):
         """states: list of dicts with ``solid`` (or ``rho_field``) and optional
-        ``seed`` / ``probability`` / ``keep_x``.  Returns the rows in order."""
+        ``seed`` / ``probability`` / ``keep_x`` / ``extra`` (keyword arguments
+        passed on to the runner's ``run_state``).  Returns the rows in order."""
         import queue
         import threading
 
@@ -345,9 +354,7 @@ class PolicySession:
         if not self._started and states:
             st0 = states[0]
             rows[0] = self._run_one(0, st0)
-            r0 = self.runners[0]
-            E0 = torch.as_tensor(r0.moduli(r0.baseline_rho_min, st0.get("solid"),
-                                           st0.get("rho_field")), device="cuda")
+            E0 = torch.as_tensor(self.runners[0].first_moduli(st0), device="cuda")
             for k in range(1, self.workers):
                 with torch.cuda.stream(self.streams[k]):
                     self.runners[k].gmg.setup(E0)

@@ -1,0 +1,156 @@
+"""Learned floor predictor on the GPU: the CNN kernels (pool, forward, BF16
+gradient, Adam) against the oracle (oracle/cnn.py), training determinism,
+and the learned policy path against the plain policy."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dtype=None):
+    import torch
+
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype)
+
+
+def _batch(B, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.random((B, 16, 8, 8)).astype(np.float32)
+    feat = rng.uniform(-1, 2, size=(B, 3)).astype(np.float32)
+    y_reg = rng.normal(size=B).astype(np.float32)
+    y_cls = rng.integers(0, 3, size=B).astype(np.int32)
+    return X, feat, y_reg, y_cls
+
+
+def _theta(seed):
+    from oracle import cnn
+
+    th = cnn.init_params(seed)
+    return (th + 0.05 * np.random.default_rng(seed + 100).normal(size=th.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("dims", [(40, 20, 20), (33, 9, 17), (160, 80, 40), (7, 3, 5)])
+def test_pool_matches_oracle(dims):
+    from gpu_fem.learned import pool_density
+    from oracle import cnn
+
+    rho = np.random.default_rng(sum(dims)).random(int(np.prod(dims)))
+    X = pool_density(rho, dims).cpu().numpy()
+    ref = cnn.adaptive_pool(rho, dims)
+    # float32 rounding of a double mean
+    assert np.abs(X - ref).max() <= 1e-7 * max(1.0, np.abs(ref).max())
+
+
+def test_forward_matches_oracle():
+    from gpu_fem.learned import FloorCNN
+    from oracle import cnn
+
+    th = _theta(1)
+    X, feat, _, _ = _batch(6, 2)
+    out = FloorCNN(theta=th).forward(_t(X), _t(feat)).cpu().numpy()
+    ref, _ = cnn.forward(th, X, feat, bf16=True)
+    err = np.abs(out - ref).max() / np.abs(ref).max()
+    print("forward rel err", err)
+    # FP32 accumulation against float64 sums of the same bf16 operands, plus
+    # rare bf16 rounding-boundary flips of intermediate activations
+    assert err <= 2e-4
+
+
+def test_gradient_matches_oracle():
+    from gpu_fem.learned import FloorCNN
+    from oracle import cnn
+
+    th = _theta(3)
+    X, feat, y_reg, y_cls = _batch(5, 4)
+    model = FloorCNN(theta=th)
+    loss, g, out = model.grad(_t(X), _t(feat), _t(y_reg), _t(y_cls), 0.8)
+    loss, g = float(loss.item()), g.cpu().numpy()
+    l_ref, g_ref, _ = cnn.loss_and_grad(th, X, feat, y_reg, y_cls, 0.8, bf16=True)
+    assert abs(loss - l_ref) <= 1e-4 * abs(l_ref)
+    for name, (o, shape) in cnn.OFFSETS.items():
+        n = int(np.prod(shape))
+        a, b = g[o:o + n], g_ref[o:o + n]
+        scale = np.abs(b).max()
+        assert scale > 0, name
+        d = np.abs(a - b) / scale
+        print(name, "max", d.max(), "median", np.median(d))
+        # every entry: within two bf16 half-ulps of the block scale (an
+        # operand whose FP32 value sits on a bf16 rounding boundary may round
+        # the other way than in the float64 oracle); almost every entry: FP32
+        # accumulation accuracy
+        assert d.max() <= 8e-3, name
+        assert np.mean(d <= 1e-4) >= 0.9, name
+
+
+def test_adam_matches_oracle():
+    from gpu_fem.learned import FloorCNN
+    from oracle import cnn
+
+    rng = np.random.default_rng(5)
+    th = rng.normal(size=cnn.N_PARAMS).astype(np.float32)
+    model = FloorCNN(theta=th)
+    t64, m64, v64 = th.astype(np.float64), np.zeros(cnn.N_PARAMS), np.zeros(cnn.N_PARAMS)
+    for k in range(1, 4):
+        g = rng.normal(size=cnn.N_PARAMS).astype(np.float32)
+        model.adam(_t(g), 1e-2)
+        t64, m64, v64 = cnn.adam_step(t64, m64, v64, g, k, 1e-2)
+    got = model.theta_host()
+    assert model.step == 3
+    assert np.abs(got - t64).max() <= 1e-5 * max(1.0, np.abs(t64).max())
+
+
+def test_training_is_deterministic_and_learns_a_separable_task():
+    """Labels that are a function of the pooled density (class by mean
+    density, regression target = 2 * mean - 1): the loss must fall and the
+    classes be recovered; two runs with the same seed give identical
+    parameters."""
+    from gpu_fem.learned import FloorCNN
+
+    rng = np.random.default_rng(11)
+    n = 192
+    level = rng.uniform(0.05, 0.95, size=n)
+    X = np.clip(level[:, None, None, None] + 0.05 * rng.normal(size=(n, 16, 8, 8)), 0, 1)
+    X = X.astype(np.float32)
+    feat = np.zeros((n, 3), np.float32)
+    y_cls = np.digitize(level, [0.35, 0.65]).astype(np.int32)
+    y_reg = (2 * level - 1).astype(np.float32)
+    thetas = []
+    for _ in range(2):
+        m = FloorCNN(seed=0)
+        hist = m.fit(_t(X), _t(feat), _t(y_reg), _t(y_cls), epochs=40, batch=32, lr=1e-2,
+                     seed=3)
+        thetas.append(m.theta_host())
+    print("loss", hist[0], hist[-1])
+    assert hist[-1] < 0.3 * hist[0]
+    cls, _ = m.predict(_t(X), _t(feat))
+    assert np.mean(cls == y_cls) >= 0.9
+    assert np.array_equal(thetas[0], thetas[1])
+
+
+def test_learned_policy_matches_plain_selected_solve_when_proposal_agrees():
+    """Proposing exactly the plain policy's accepted floor reproduces its
+    selected solve: same iteration count and compliance (one session each,
+    same first set-up moduli, so the cached spectral estimate is equal)."""
+    from gpu_fem.learned import (FLOORS, LearnedFloorPolicy, POLICY_DEFAULTS, floor_class,
+                                 label_states, sample_states)
+    from gpu_fem.policy import PolicySession
+    from gpu_fem.presets import cantilever
+
+    states = sample_states(4, 7, [(24, 12, 12)], kinds=("bernoulli",), p_range=(0.3, 0.5))
+    for st in states:
+        row = label_states([st], workers=1)[0]  # a session of its own
+        k = floor_class(row["recommended_rho_min"])
+        if k != 0:
+            continue  # the plain selected solve follows a probe at another floor
+        session = PolicySession(cantilever(*st["dims"], load="edge"), workers=1,
+                                runner_cls=LearnedFloorPolicy, **POLICY_DEFAULTS)
+        rho = (np.random.default_rng(st["seed"]).random(int(np.prod(st["dims"])))
+               < st["probability"])
+        got = session.run([{"solid": rho, "extra": {"predicted_class": k}}])[0]
+        assert got["guard_fallback"] == 0
+        assert got["solve_iters"] == row["solve_iters"]
+        assert got["recommended_rho_min"] == FLOORS[k]
+        assert abs(got["compliance"] - row["compliance"]) <= 1e-12 * abs(row["compliance"])
+        return
+    pytest.skip("no state kept the baseline floor")

@@ -125,3 +125,65 @@ def test_density_csv_schema_and_history_rows(tmp_path):
     dfm.write_csv(str(q), h, dfm.DENSITY_HISTORY_FIELDS)
     with open(q) as f:
         assert next(csv.reader(f)) == dfm.DENSITY_HISTORY_FIELDS
+
+
+# ---- learned-floor form: host logic ----
+def test_learned_ladder_rules():
+    from gpu_fem.learned import learned_ladder
+
+    floors = (1e-12, 1e-3, 1e-2)
+    verdicts = {1e-12: False, 1e-3: True, 1e-2: True}
+    assert learned_ladder(1, floors, verdicts.get) == ([1e-3], 1, False)
+    assert learned_ladder(0, floors, verdicts.get) == ([1e-12, 1e-3], 1, True)
+    assert learned_ladder(2, floors, verdicts.get) == ([1e-2], 2, False)
+    assert learned_ladder(0, floors, lambda r: False) == ([1e-12, 1e-3, 1e-2], None, True)
+    with pytest.raises(ValueError):
+        learned_ladder(3, floors, verdicts.get)
+
+
+def test_learned_targets_and_classes():
+    from gpu_fem.learned import FLOORS, floor_class, grid_features, r50_from_target, r50_target
+
+    r = np.array([1e-9, 3.2e-5, 0.7])
+    assert np.allclose(r50_from_target(r50_target(r)), r, rtol=1e-12)
+    # breakdown (0), divergence (inf) and NaN probes are clipped
+    assert np.allclose(r50_target([0.0, np.inf, np.nan, 1e-20]),
+                       [(-12 + 5) / 3, (0.5 + 5) / 3, (0.5 + 5) / 3, (-12 + 5) / 3])
+    assert [floor_class(f) for f in FLOORS] == [0, 1, 2]
+    assert floor_class(1.0000001e-3) == 1
+    assert np.allclose(grid_features((32, 64, 16)), [0.0, 1.0, -1.0])
+
+
+def test_learned_init_matches_oracle_initialiser():
+    from gpu_fem.learned import init_theta
+    from oracle import cnn
+
+    assert np.array_equal(init_theta(4), cnn.init_params(4))
+
+
+def test_learned_state_sampling_is_seeded():
+    from gpu_fem.learned import density_of, sample_states
+
+    grids = [(40, 20, 20), (30, 15, 15)]
+    a = sample_states(6, 3, grids)
+    assert a == sample_states(6, 3, grids)
+    assert [s["kind"] for s in a] == ["bernoulli", "sigmoid"] * 3
+    d0, d1 = density_of(a[0]), density_of(a[1])
+    assert set(np.unique(d0)) <= {0.0, 1.0}
+    assert abs(d1.mean() - a[1]["volfrac"]) < 1e-6 and 0.0 < d1.min() and d1.max() <= 1.0
+    assert 0.05 < np.mean((d1 > 0.05) & (d1 < 0.95)) < 0.95  # intermediate densities
+
+
+def test_learned_compare_counts():
+    from gpu_fem.learned import compare
+
+    plain = [{"recommended_rho_min": 1e-12, "solve_converged": 1, "probe_r50": 1e-6},
+             {"recommended_rho_min": 1e-3, "solve_converged": 1, "probe_r50": 1e-2}]
+    learned = [{"recommended_rho_min": 1e-12, "predicted_class": 0, "guard_fallback": 0,
+                "solve_converged": 1, "solve_final_rel_residual": 5e-7, "predicted_r50": 1e-6},
+               {"recommended_rho_min": 1e-2, "predicted_class": 2, "guard_fallback": 0,
+                "solve_converged": 1, "solve_final_rel_residual": 9e-7, "predicted_r50": 1e-1}]
+    c = compare(plain, learned)
+    assert c["floor_agreement"] == 0.5 and c["proposal_agreement"] == 0.5
+    assert c["fallback_rate"] == 0.0 and c["max_accepted_true_rel_residual"] == 9e-7
+    assert abs(c["r50_log10_rmse"] - np.sqrt(0.5)) < 1e-12
