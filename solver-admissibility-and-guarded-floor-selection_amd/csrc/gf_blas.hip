@@ -571,6 +571,36 @@ std::recursive_mutex& device_mutex() {
   return mu;
 }
 
+static hipStream_t init_stream(int* rc) {
+  static hipStream_t s = nullptr;  // guarded by the device lock
+  *rc = 0;
+  if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreateWithFlags failed for the set-up stream");
+    *rc = -1;
+  }
+  return s;
+}
+
+int upload_now(void* dst, const void* src, size_t bytes) {
+  GF_DEVICE_LOCK();
+  int rc = 0;
+  hipStream_t s = init_stream(&rc);
+  if (rc) return rc;
+  GF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  GF_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int zero_now(void* dst, size_t bytes) {
+  GF_DEVICE_LOCK();
+  int rc = 0;
+  hipStream_t s = init_stream(&rc);
+  if (rc) return rc;
+  GF_HIP(hipMemsetAsync(dst, 0, bytes, s));
+  GF_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 int Work::init() {
   if (partials) return 0;
   GF_DEVICE_LOCK();
@@ -587,10 +617,10 @@ int Work::init() {
   GF_HIP(hipExtMallocWithFlags((void**)&partials, sizeof(double) * 4 * kRedBlocks,
                                hipDeviceMallocUncached));
   GF_HIP(hipExtMallocWithFlags((void**)&ticket, sizeof(int) * 64, hipDeviceMallocUncached));
-  GF_HIP(hipMemset(ticket, 0, sizeof(int) * 64));
+  GF_TRY(zero_now(ticket, sizeof(int) * 64));
   GF_HIP(hipExtMallocWithFlags((void**)&flags, sizeof(int) * kSweepMaxBlocks,
                                hipDeviceMallocUncached));
-  GF_HIP(hipMemset(flags, 0, sizeof(int) * kSweepMaxBlocks));
+  GF_TRY(zero_now(flags, sizeof(int) * kSweepMaxBlocks));
   sweep_G = -1;
   return 0;
 }

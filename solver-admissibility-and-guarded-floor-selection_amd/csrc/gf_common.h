@@ -119,6 +119,15 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 std::recursive_mutex& device_mutex();
 #define GF_DEVICE_LOCK() std::lock_guard<std::recursive_mutex> _gf_dev_lock(::gf::device_mutex())
 
+// Set-up writes of device memory that later work on any stream reads (host
+// tables, zeroed flags and scalars).  They run on a private non-blocking
+// stream and are waited for before returning, so they are complete before
+// the caller enqueues anything: a null-stream hipMemset/hipMemcpy is not
+// ordered with the non-blocking streams the solvers run on, and a late one
+// can land in the middle of a solve.
+int upload_now(void* dst, const void* src, size_t bytes);
+int zero_now(void* dst, size_t bytes);
+
 // Wait until every memory operation this lane issued has been acknowledged
 // (s_waitcnt with all counters at zero; on gfx9 stores count in vmcnt).  The
 // signal fences keep the compiler from moving memory accesses across it.

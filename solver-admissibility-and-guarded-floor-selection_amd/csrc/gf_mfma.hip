@@ -103,7 +103,7 @@ int Ctx::bf16_alloc() {
     for (int j = 0; j < 24; ++j) h[i * 32 + j] = bf16_bits_rne((float)h_Ke[i * 24 + j]);
   const long n_e = (long)g.nx * g.ny * g.nz;
   GF_HIP(hipMalloc(&KEb, sizeof(h)));
-  GF_HIP(hipMemcpy(KEb, h, sizeof(h), hipMemcpyHostToDevice));
+  GF_TRY(upload_now(KEb, h, sizeof(h)));
   GF_HIP(hipMalloc(&Fel, sizeof(float) * 24 * (size_t)n_e));
   return 0;
 }

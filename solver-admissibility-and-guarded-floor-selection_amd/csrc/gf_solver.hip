@@ -337,13 +337,13 @@ int Ctx::init(int nelx, int nely, int nelz, const double* Ke_host, const int* fr
   off = (h_free[0] + nfree == ndof) ? h_free[0] : -1;  // sorted+unique => contiguous tail
   std::memcpy(h_Ke, Ke_host, sizeof(double) * 576);
   GF_TRY(dalloc(&d_f2f, ndof));
-  GF_HIP(hipMemcpy(d_f2f, h_f2f.data(), sizeof(int) * ndof, hipMemcpyHostToDevice));
+  GF_TRY(upload_now(d_f2f, h_f2f.data(), sizeof(int) * ndof));
   GF_TRY(dalloc(&Ke, 576));
   GF_TRY(dalloc(&Ke32, 576));
   float k32[576];
   for (int i = 0; i < 576; ++i) k32[i] = (float)h_Ke[i];
-  GF_HIP(hipMemcpy(Ke, h_Ke, sizeof(double) * 576, hipMemcpyHostToDevice));
-  GF_HIP(hipMemcpy(Ke32, k32, sizeof(float) * 576, hipMemcpyHostToDevice));
+  GF_TRY(upload_now(Ke, h_Ke, sizeof(double) * 576));
+  GF_TRY(upload_now(Ke32, k32, sizeof(float) * 576));
   GF_TRY(work.init());
   return 0;
 }
@@ -353,8 +353,8 @@ int Ctx::nodeblock_alloc() {
   if (nb_inv9) return 0;
   GF_TRY(dalloc(&nb_inv9, 9L * g.n_nodes));
   GF_TRY(dalloc(&nb_pinv, n_free));
-  GF_HIP(hipMemset(nb_inv9, 0, sizeof(double) * 9L * g.n_nodes));
-  GF_HIP(hipMemset(nb_pinv, 0, sizeof(double) * n_free));
+  GF_TRY(zero_now(nb_inv9, sizeof(double) * 9L * g.n_nodes));
+  GF_TRY(zero_now(nb_pinv, sizeof(double) * n_free));
   return 0;
 }
 
@@ -434,8 +434,8 @@ int Krylov::alloc(int m_, long n_) {
   GF_TRY(dalloc(&y, m));
   GF_TRY(dalloc(&sc, 8));
   GF_TRY(dalloc(&fl, 8));
-  GF_HIP(hipMemset(fl, 0, sizeof(int) * 8));
-  GF_HIP(hipMemset(sc, 0, sizeof(double) * 8));
+  GF_TRY(zero_now(fl, sizeof(int) * 8));
+  GF_TRY(zero_now(sc, sizeof(double) * 8));
   return 0;
 }
 
@@ -585,7 +585,7 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
     std::vector<int> f2f(lv[l].g.ndof, -1);
     for (int i = 0; i < nf; ++i) f2f[lv[l].h_free[i]] = i;
     GF_TRY(dalloc(&lv[l].d_f2f, lv[l].g.ndof));
-    GF_HIP(hipMemcpy(lv[l].d_f2f, f2f.data(), sizeof(int) * f2f.size(), hipMemcpyHostToDevice));
+    GF_TRY(upload_now(lv[l].d_f2f, f2f.data(), sizeof(int) * f2f.size()));
     GF_TRY(dalloc(&lv[l].A, 243L * lv[l].g.n_nodes));
   }
   for (int l = 0; l < L; ++l) {
@@ -629,7 +629,7 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
   std::vector<double> Qh;
   galerkin_q(ctx->h_Ke, Qh);
   GF_TRY(dalloc(&Q, Qh.size()));
-  GF_HIP(hipMemcpy(Q, Qh.data(), sizeof(double) * Qh.size(), hipMemcpyHostToDevice));
+  GF_TRY(upload_now(Q, Qh.data(), sizeof(double) * Qh.size()));
   GF_TRY(dalloc(&d_fail, 4));
   GF_TRY(dalloc(&ap_x, n0));
   GF_TRY(dalloc(&ap_res, n0));
@@ -645,7 +645,7 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
   GF_TRY(dalloc(&pw_s, 1024));
   GF_HIP(hipHostMalloc((void**)&hs, sizeof(double) * 16, hipHostMallocDefault));
   GF_TRY(dalloc(&d_err, 1));
-  GF_HIP(hipMemset(d_err, 0, sizeof(double)));
+  GF_TRY(zero_now(d_err, sizeof(double)));
   if (p.root_enrichment_mode != 0) GF_TRY(dalloc(&re_w, n0));
   if (p.inner_krylov_steps > 0) GF_TRY(ik.alloc(p.inner_krylov_restart, n0));
   if (p.root_local_correction_mode == 2)
@@ -653,8 +653,8 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
   if (p.root_local_correction_mode != 0) {
     GF_TRY(dalloc(&nb_inv9, 9L * ctx->g.n_nodes));
     GF_TRY(dalloc(&nb_pinv, n0));
-    GF_HIP(hipMemset(nb_inv9, 0, sizeof(double) * 9L * ctx->g.n_nodes));
-    GF_HIP(hipMemset(nb_pinv, 0, sizeof(double) * n0));
+    GF_TRY(zero_now(nb_inv9, sizeof(double) * 9L * ctx->g.n_nodes));
+    GF_TRY(zero_now(nb_pinv, sizeof(double) * n0));
   }
   lam.assign(L, NAN);
   GF_TRY(work.init());
