@@ -38,7 +38,14 @@ FP64_PEAK_TFLOPS = 78.6
 # (profiles/fine_matvec_fp64_80x40x20_roofline.md)
 TRAFFIC_BYTES = 4309925
 SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
-                 8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
+                 8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>",
+                 82: "k_fine_matvec_zblk<TZ=8,R=2>", 83: "k_fine_matvec_zblk<TZ=8,R=3>",
+                 42: "k_fine_matvec_zblk<TZ=4,R=2>"}
+
+
+def progress(msg):
+    """One progress line on stderr (the JSON result line stays alone on stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -61,10 +68,29 @@ def parse():
                     help="HIP hardware queues for this process (GPU_MAX_HW_QUEUES, <= 32; "
                          "0 keeps the runtime default)")
     ap.add_argument("--states-csv", default="")
-    ap.add_argument("--form", default="policy", choices=["policy", "batched"],
-                    help="policy: the guarded policy on configs[1] (default line); batched: "
-                         "fixed-floor guarded solves of a mixed-size batch, FP32 vs FP64 "
-                         "preconditioner")
+    ap.add_argument("--form", default="policy",
+                    choices=["policy", "batched", "precision", "learned"],
+                    help="policy: the plain FP64 guarded policy on the 80x40x20 Bernoulli "
+                         "states (default line); batched: fixed-floor guarded solves of a "
+                         "mixed-size batch, FP32 vs FP64 preconditioner; precision: one "
+                         "filtered-sigmoid state through the policy with an FP32 vs FP64 "
+                         "multigrid (use --grid 200x100x50); learned: train the floor "
+                         "network and compare learned + FP32 multigrid with the plain policy")
+    ap.add_argument("--density-seed", type=int, default=0)
+    ap.add_argument("--floors", default="1e-12,1e-3",
+                    help="precision form: probe floor, then the raised ladder")
+    ap.add_argument("--probe-iters", type=int, default=100)
+    ap.add_argument("--restart", type=int, default=300)
+    ap.add_argument("--maxiter", type=int, default=300)
+    ap.add_argument("--train-states", type=int, default=768)
+    ap.add_argument("--train-grids", default="40x20x20,64x32x16,80x40x20")
+    ap.add_argument("--train-seed", type=int, default=1)
+    ap.add_argument("--eval-states", type=int, default=96)
+    ap.add_argument("--eval-seed", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--labels", default="", help="label CSV to train from (skips labelling)")
+    ap.add_argument("--labels-out", default="", help="write the training labels here")
+    ap.add_argument("--theta-out", default="", help="save the trained parameters (.npy)")
     ap.add_argument("--batch-states", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the FP64 fine matvec loop (for rocprofv3 runs)")
@@ -212,7 +238,9 @@ def batched_form(args):
     for prec in ("fp32", "fp64"):
         solver = BatchedFloorSolver(workers=args.workers, restart=100, maxiter=300, tol=1e-6,
                                     penal=args.penal, load=args.load, precision=prec)
+        progress(f"batched {prec}: warm-up")
         solver.run(states[: min(len(states), args.workers)])      # warm-up (kernels, pools)
+        progress(f"batched {prec}: timed batch of {len(states)} states")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rows = solver.run(states)
@@ -256,6 +284,205 @@ def batched_form(args):
     print(json.dumps(line))
 
 
+def precision_form(args):
+    """Precision form: one filtered-sigmoid density state (seeded N(0,1) noise,
+    cone filter radius 3, sigmoid sharpness 8, volume fraction 0.3) on the
+    cantilever grid (default 200x100x50, 1,000,000 hex8), taken through the
+    guarded policy with floors {1e-12 probe, 1e-3}, FGMRES restart 300, tol
+    1e-6: once with the plain FP64 preconditioner and once with every
+    multigrid level in FP32 (FP32-stored Galerkin stencils, FP32 smoothers;
+    the outer FGMRES, the guard residual and the compliance stay FP64).
+    Reports both forms' accepted solves/s and the FP32 result's compliance and
+    solution error against the FP64 result (the FP64 form is the one held to
+    the oracle), plus the fine-matvec rate at this grid."""
+    import numpy as np
+    import torch
+
+    from gpu_fem import policy as gpol
+    from gpu_fem.batch import precision_stack
+    from gpu_fem.density_field import filtered_sigmoid_density
+    from gpu_fem.presets import cantilever
+
+    dims = tuple(int(v) for v in args.grid.split("x"))
+    n_e = dims[0] * dims[1] * dims[2]
+    floors = tuple(float(v) for v in args.floors.split(","))
+    progress(f"precision: density {dims}")
+    rho = filtered_sigmoid_density(args.density_seed, dims, radius=3.0, sharpness=8.0,
+                                   volfrac=0.3)
+    spec = cantilever(*dims, load=args.load)
+    kw = dict(penal=args.penal, baseline_rho_min=1e-12, raised_rho_mins=floors[1:],
+              probe_iters=args.probe_iters, probe_r50=min(50, args.probe_iters),
+              restart=args.restart, maxiter=args.maxiter, tol=1e-6, mgs_sweep=args.mgs_sweep,
+              preset_name=f"cantilever_{dims[0]}x{dims[1]}x{dims[2]}_{args.load}")
+    forms, xs, roof = {}, {}, {}
+    for prec in ("fp64", "fp32"):
+        progress(f"precision {prec}: build")
+        runner = gpol.GuardedFloorPolicy(spec, **kw, **precision_stack(prec))
+        torch.cuda.synchronize()
+        progress(f"precision {prec}: guarded policy")
+        t0 = time.perf_counter()
+        row = runner.run_state(rho_field=rho, keep_x=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        xs[prec] = row.pop("x")
+        row["wall_s"] = wall
+        forms[prec] = row
+        if prec == "fp64":
+            avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner)
+            roof = {"avg_launch_us": avg_us, "algorithmic_bytes": bytes_alg, "flops": flops,
+                    "hbm_GB_s": gbs, "fp64_tflops": tfs,
+                    "launch_shape": runner.mf_op.launch_shape}
+        progress(f"precision {prec}: {wall:.1f} s, floor {row['recommended_rho_min']:.0e}, "
+                 f"iters {row['solve_iters']}")
+        del runner
+        torch.cuda.empty_cache()
+    f64, f32 = forms["fp64"], forms["fp32"]
+    x64, x32 = xs["fp64"], xs["fp32"]
+    sol_err = float(torch.linalg.vector_norm(x32 - x64) / torch.linalg.vector_norm(x64))
+
+    def brief(r):
+        return {k: r[k] for k in ("trigger", "recommended_rho_min", "attempted_rho_mins",
+                                  "probe_r50", "probe_r100", "solve_converged", "solve_iters",
+                                  "solve_final_rel_residual", "compliance", "wall_s",
+                                  "probe_setup_time_s", "probe_solve_time_s",
+                                  "selected_setup_time_s", "selected_solve_time_s")}
+
+    v32 = f32["solve_converged"] / f32["wall_s"]
+    v64 = f64["solve_converged"] / f64["wall_s"]
+    line = {
+        "metric": "accepted guarded solves/s, precision form (FP32 multigrid) vs plain FP64",
+        "value": v32, "unit": "accepted guarded solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": f32["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 outer / f32 preconditioner",
+        "data": "synthetic (seeded filtered-sigmoid density)",
+        "config": {
+            "workload": (f"cantilever {dims[0]}x{dims[1]}x{dims[2]} ({n_e} hex8), filtered-sigmoid "
+                         f"density seed {args.density_seed} (radius 3, sharpness 8, volume "
+                         f"fraction 0.3), SIMP p={args.penal:g}, {args.load} load, floors "
+                         f"{{{args.floors}}}, probe {args.probe_iters}, restart {args.restart}, "
+                         f"maxiter {args.maxiter}, tol 1e-6"),
+            "n_free": int(x64.numel()),
+            "plain_fp64_value": v64,
+            "speedup_vs_plain_fp64": v32 / max(v64, 1e-300),
+            "fp32": brief(f32), "fp64": brief(f64),
+            "floor_decisions_agree": int(f32["recommended_rho_min"] == f64["recommended_rho_min"]
+                                         and f32["trigger"] == f64["trigger"]),
+            "compliance_rel_err_vs_fp64": abs(f32["compliance"] - f64["compliance"])
+            / abs(f64["compliance"]),
+            "solution_rel_err_vs_fp64": sol_err,
+            "fp32_true_rel_residual_fp64_recomputed": f32["solve_final_rel_residual"],
+            "fine_matvec_fp64": roof,
+            "parallelism": "single GPU, one solver instance",
+        },
+    }
+    print(json.dumps(line))
+
+
+def learned_form(args):
+    """Learned-floor form.  Training states (Bernoulli and filtered-sigmoid
+    densities on the --train-grids) are labelled by the plain FP64 guarded
+    policy -- or read from --labels, a CSV an earlier labelling run wrote --
+    and the 3D CNN is trained on the GPU by the library's own BF16 kernels.
+    Fresh-seed evaluation states on --grid then run three ways, each as
+    concurrent solver instances: the plain FP64 policy (probe + ladder +
+    guard), the learned policy with the FP64 multigrid, and the learned policy
+    with the FP32 multigrid.  ``value`` is the last one's accepted guarded
+    solves/s; the plain form's rate, the speed-ups, the floor agreement with
+    the plain ladder and the guard fallback rate are reported beside it."""
+    import numpy as np
+    import torch
+
+    from gpu_fem import learned as L
+    from gpu_fem.batch import precision_stack
+
+    pk = dict(penal=args.penal)
+    grids = [tuple(int(v) for v in g.split("x")) for g in args.train_grids.split(",") if g]
+    train = L.sample_states(args.train_states, args.train_seed, grids)
+    t_label = None
+    if args.labels and os.path.exists(args.labels):
+        progress(f"learned: reading labels {args.labels}")
+        train, rows = L.read_labels(args.labels)
+    else:
+        progress(f"learned: labelling {len(train)} training states with the plain policy")
+        t0 = time.perf_counter()
+        rows = L.label_states(train, workers=args.workers, load=args.load, **pk)
+        t_label = time.perf_counter() - t0
+        if args.labels_out:
+            L.write_labels(args.labels_out, train, rows)
+    progress("learned: pooling training inputs")
+    L.attach_pooled(train, rows)
+    X, feat, y_reg, y_cls = L.training_tensors(rows)
+    model = L.FloorCNN(seed=0)
+    progress(f"learned: training ({args.epochs} epochs)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = model.fit(X, feat, y_reg, y_cls, epochs=args.epochs, batch=64, lr=3e-3, seed=0)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    cls_tr, _ = model.predict(X, feat)
+    train_acc = float(np.mean(cls_tr == y_cls.cpu().numpy()))
+    if args.theta_out:
+        model.save(args.theta_out)
+
+    dims = tuple(int(v) for v in args.grid.split("x"))
+    ev = L.sample_states(args.eval_states, args.eval_seed, [dims])
+    dens = [L.density_of(s) for s in ev]
+    # warm-up: kernels and pools of this grid (one state each way)
+    progress("learned: warm-up")
+    L.label_states(ev[:1], workers=1, load=args.load, densities=dens[:1], **pk)
+    progress(f"learned: plain FP64 policy on {len(ev)} evaluation states")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = L.label_states(ev, workers=args.workers, load=args.load, densities=dens, **pk)
+    torch.cuda.synchronize()
+    w_plain = time.perf_counter() - t0
+    forms = {}
+    for prec in ("fp64", "fp32"):
+        progress(f"learned: learned policy, {prec} multigrid")
+        got, wall = L.run_learned(model, ev, workers=args.workers, load=args.load,
+                                  densities=dens, **pk, **precision_stack(prec))
+        forms[prec] = (got, wall)
+
+    def acc(rs):
+        return sum(int(r["solve_converged"]) for r in rs)
+
+    v_plain = acc(plain) / w_plain
+    out = {}
+    for prec, (got, wall) in forms.items():
+        c = L.compare(plain, got)
+        c.update(value=acc(got) / wall, wall_s=wall, speedup_vs_plain_fp64=(acc(got) / wall) / v_plain,
+                 mean_total_iters=float(np.mean([r["total_iters"] for r in got])))
+        out[prec] = c
+    r32, w32 = forms["fp32"]
+    plain_iters = [len(r["probe_history"]) - 1 + sum(len(h) - 1 for _, h in r["solve_history"])
+                   for r in plain]
+    line = {
+        "metric": ("accepted guarded solves/s, learned floor + FP32 multigrid + concurrent "
+                   "instances vs plain FP64"),
+        "value": out["fp32"]["value"], "unit": "accepted guarded solves/s", "n_gpus": 1,
+        "steps": len(ev), "warmup": 1, "ms_per_step": w32 / len(ev) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 outer / f32 preconditioner; network bf16 (fp32 accumulation)",
+        "data": "synthetic (seeded Bernoulli and filtered-sigmoid density states)",
+        "config": {
+            "workload": (f"{len(ev)} fresh-seed states on cantilever {dims[0]}x{dims[1]}x{dims[2]} "
+                         f"(Bernoulli p in [0.1,0.5] and filtered-sigmoid volume fraction in "
+                         f"[0.2,0.5]), SIMP p={args.penal:g}, {args.load} load, ladder "
+                         "{1e-12,1e-3,1e-2}, restart 300, tol 1e-6"),
+            "training": {"states": len(train), "grids": [list(g) for g in grids],
+                         "labelling_s": t_label, "train_s": t_train, "epochs": args.epochs,
+                         "loss_first_last": [hist[0], hist[-1]], "train_class_accuracy": train_acc,
+                         "labels": args.labels or None},
+            "plain_fp64": {"value": v_plain, "wall_s": w_plain, "accepted": acc(plain),
+                           "mean_total_iters": float(np.mean(plain_iters))},
+            "learned_fp64_multigrid": out["fp64"],
+            "learned_fp32_multigrid": out["fp32"],
+            "parallelism": f"single GPU, {args.workers} concurrent solver instances",
+        },
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.hw_queues:
@@ -275,6 +502,12 @@ def main():
 
     if args.form == "batched":
         batched_form(args)
+        return
+    if args.form == "precision":
+        precision_form(args)
+        return
+    if args.form == "learned":
+        learned_form(args)
         return
     dims = tuple(int(v) for v in args.grid.split("x"))
     from gpu_fem.presets import cantilever
@@ -319,7 +552,9 @@ def main():
     # warm-up: the session's first state (sets the cached lambda for every
     # instance) and then one state per instance
     if args.warmup > 0:
+        progress("policy: warm-up states")
         session.run(batch(max(args.warmup, args.workers)))
+    progress(f"policy: {args.steps} timed states")
     torch.cuda.synchronize()
     timed = batch(args.steps)
     t0 = time.perf_counter()
@@ -343,6 +578,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        progress("cpu baseline (oracle) sample")
         cpu = cpu_baseline(args, dims, rows)
 
     out = {

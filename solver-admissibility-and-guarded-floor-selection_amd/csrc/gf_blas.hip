@@ -6,7 +6,7 @@
 // and never needs a host round trip.  All kernels accept an optional `gate`
 // (device int); when it points at 0 the kernel is a no-op.  The inner Krylov
 // loops use this to stop after a breakdown without host synchronisation.
-#include "gf_internal.h"
+#include "gf_reduce.h"
 
 #include <atomic>
 #include <mutex>
@@ -19,76 +19,6 @@ std::string& err_slot() {
 }
 void set_error(const std::string& msg) { err_slot() = msg; }
 const char* last_error() { return err_slot().c_str(); }
-
-// ---------------------------------------------------------------------------
-// block reduction helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-
-// Sum over a 256-thread block; result valid in thread 0.
-__device__ __forceinline__ double block_sum(double v, double* lds) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) lds[wid] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-  __syncthreads();
-  return r;
-}
-
-// Fold NP partial sums of every block into out[0..NP) (last block only).
-// Returns true in the block that performed the fold.
-template <int NP>
-__device__ __forceinline__ void finish_reduce(const double (&v)[NP], double* partials,
-                                              int* ticket, double* out, int post) {
-  __shared__ double lds[4];
-  __shared__ bool last;
-  double bs[NP];
-#pragma unroll
-  for (int p = 0; p < NP; ++p) bs[p] = block_sum(v[p], lds);
-  // partials and ticket live in uncached device memory, so a partial is
-  // visible to every XCD once its store is acknowledged.  Waiting for that
-  // acknowledgement (vmcnt 0) before the ticket increment is the part of an
-  // agent-scope release that matters here; the L2 write-back that such a
-  // release would add is not needed for memory that L2 does not hold.
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int p = 0; p < NP; ++p)
-      __hip_atomic_store(&partials[p * kRedBlocks + blockIdx.x], bs[p], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    store_ack_wait();
-    const int t = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  const int nb = gridDim.x;
-#pragma unroll
-  for (int p = 0; p < NP; ++p) {
-    double a = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x)
-      a += __hip_atomic_load(&partials[p * kRedBlocks + i], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    double s = block_sum(a, lds);
-    if (threadIdx.x == 0) {
-      if (post == 1) s = sqrt(s);
-      out[p] = s;
-    }
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Reduction grid: ~4096 elements per block, between 16 and kRedBlocks blocks
-// (measured on MI355X: fewer, fuller blocks shorten the ticket tail).
-inline int red_grid(long n) {
-  int g = cdiv(n, 4096);
-  return g < 16 ? 16 : (g > kRedBlocks ? kRedBlocks : g);
-}
 
 // ---------------------------------------------------------------------------
 // kernels

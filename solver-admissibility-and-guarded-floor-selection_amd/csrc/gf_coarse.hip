@@ -6,7 +6,7 @@
 // (SoA layout, see gf_common.h).  Entries whose row or column DOF is fixed are
 // stored as exact zeros, which reproduces the reference's free-restricted
 // CSR matrices value for value.
-#include "gf_internal.h"
+#include "gf_reduce.h"
 
 #include <atomic>
 
@@ -840,98 +840,160 @@ int dense_recip_diag(int n, const double* L, double* rdiag, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// fixed-count Jacobi PCG on the coarsest stencil (one persistent work-group;
-// the reference's device-scalar loop, solver_v2.py:781-810)
+// Fixed-count Jacobi PCG on the coarsest stencil, zero start, no stopping
+// test (the reference's device-scalar loop, solver_v2.py:781-810), in two
+// grid-wide launches per iteration with every scalar in device memory (the
+// loop is stream-ordered and is recorded in the preconditioner graph):
+//   k_pcg_spmv  : p_it = dinv r + beta p_{it-1} (beta from s), formed on the
+//                 fly for every stencil neighbour and stored for owned DOFs
+//                 in the buffer of this iteration's parity; Ap = A p_it;
+//                 s[2] = p_it . Ap
+//   k_pcg_update: alpha = s[cur] / (s[2] + 1e-300); x += alpha p_it;
+//                 r -= alpha Ap; s[cur^1] = r . (dinv r)
+// The direction is one expression (pcg_dir) evaluated identically wherever a
+// DOF's value is needed, so every block sees the same bits.  s[0], s[1] hold
+// r.z of alternate iterations.
 // ---------------------------------------------------------------------------
-constexpr int kPcgThreads = 1024;
-
-__device__ double wg_sum(double v, double* lds) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) lds[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += lds[i];
-    lds[16] = s;
-  }
-  __syncthreads();
-  return lds[16];
+__device__ __forceinline__ double pcg_dir(double dinv, double r, double beta, double pold) {
+  return __fma_rn(beta, pold, __dmul_rn(dinv, r));
 }
 
-__global__ __launch_bounds__(kPcgThreads) void k_fixed_pcg(Grid g, DofMap map,
-                                                           const double* __restrict__ A,
-                                                           const double* __restrict__ dinv,
-                                                           int nf, int iters,
-                                                           const double* __restrict__ b,
-                                                           double* __restrict__ x,
-                                                           double* __restrict__ r,
-                                                           double* __restrict__ p,
-                                                           double* __restrict__ Ap,
-                                                           const int* gate) {
+__global__ __launch_bounds__(kBlock) void k_pcg_init(long n, const double* __restrict__ b,
+                                                     const double* __restrict__ dinv,
+                                                     double* __restrict__ x,
+                                                     double* __restrict__ r, double* partials,
+                                                     int* ticket, double* s, const int* gate) {
   if (gate && *gate == 0) return;
-  __shared__ double lds[17];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+  double a[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double bi = b[i];
     x[i] = 0.0;
-    r[i] = b[i];
-    const double z = dinv[i] * b[i];
-    p[i] = z;
-    acc += b[i] * z;
+    r[i] = bi;
+    a[0] += bi * __dmul_rn(dinv[i], bi);
   }
-  double rz = wg_sum(acc, lds);
-  const long nn = g.n_nodes;
-  for (int it = 0; it < iters; ++it) {
-    __syncthreads();
-    double pap = 0.0;
-    for (int n = threadIdx.x; n < g.n_nodes; n += blockDim.x) {
-      const int o[3] = {map(3 * n), map(3 * n + 1), map(3 * n + 2)};
-      if ((o[0] & o[1] & o[2]) < 0) continue;
-      const int iz = n % g.NZ, t = n / g.NZ, iy = t % g.NY, ix = t / g.NY;
-      double a0 = 0, a1 = 0, a2 = 0;
-      for (int k = 0; k < 27; ++k) {
-        const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
-        if (jx < 0 || jx >= g.NX || jy < 0 || jy >= g.NY || jz < 0 || jz >= g.NZ) continue;
-        const int m = (jx * g.NY + jy) * g.NZ + jz;
-        for (int q = 0; q < 3; ++q) {
-          const int f = map(3 * m + q);
-          if (f < 0) continue;
-          const double pv = p[f];
-          a0 += A[(k * 9 + q) * nn + n] * pv;
-          a1 += A[(k * 9 + 3 + q) * nn + n] * pv;
-          a2 += A[(k * 9 + 6 + q) * nn + n] * pv;
-        }
-      }
-      const double av[3] = {a0, a1, a2};
-      for (int q = 0; q < 3; ++q)
-        if (o[q] >= 0) Ap[o[q]] = av[q];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) pap += p[i] * Ap[i];
-    const double pAp = wg_sum(pap, lds);
-    const double alpha = rz / (pAp + 1e-300);
-    double rzn = 0.0;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
-      x[i] = x[i] + alpha * p[i];
-      const double ri = r[i] - alpha * Ap[i];
-      r[i] = ri;
-      rzn += ri * (dinv[i] * ri);
-    }
-    const double rz_n = wg_sum(rzn, lds);
-    const double beta = rz_n / (rz + 1e-300);
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) p[i] = dinv[i] * r[i] + beta * p[i];
-    rz = rz_n;
-  }
+  finish_reduce<1>(a, partials, ticket, s, 0);
 }
 
-int fixed_pcg(const Grid& g, const DofMap& map, const double* A, const double* dinv, int n_free,
-              int iters, const double* b, double* x, double* work, const int* gate,
-              hipStream_t st) {
-  hipLaunchKernelGGL(k_fixed_pcg, dim3(1), dim3(kPcgThreads), 0, st, g, map, A, dinv, n_free,
-                     iters, b, x, work, work + n_free, work + 2L * n_free, gate);
+// FIRST: p_0 = dinv b (= dinv r_0, beta = 0).  64 nodes x 4 waves per block
+// as k_stencil_nodes; the slot products are folded in slot order 0..26.
+template <bool OFF, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(Grid g, DofMap map,
+                                                     const double* __restrict__ A,
+                                                     const double* __restrict__ dinv,
+                                                     const double* __restrict__ r,
+                                                     const double* __restrict__ pold,
+                                                     double* __restrict__ pnew,
+                                                     double* __restrict__ Ap, double* partials,
+                                                     int* ticket, double* s, int cur,
+                                                     const int* gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double tk[27][3][64];
+  const double beta = FIRST ? 0.0 : s[cur] / (s[cur ^ 1] + 1e-300);
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = blockIdx.x * 64 + lane;
+  const bool valid = n < g.n_nodes;
+  const int nn_ = valid ? n : 0;
+  int ix, iy, iz;
+  node_xyz(g, nn_, ix, iy, iz);
+  const long nn = g.n_nodes;
+  for (int k = wv; k < 27; k += 4) {
+    const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+    const bool in = valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
+                    (jz < g.NZ);
+    const int m = (jx * g.NY + jy) * g.NZ + jz;
+    double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int f = in ? map_idx<OFF>(map, 3 * m + q) : -1;
+      const int fi = f >= 0 ? f : 0;
+      const double dv = dinv[fi], rv = r[fi];
+      const double pv = FIRST ? __dmul_rn(dv, rv) : pcg_dir(dv, rv, beta, pold[fi]);
+      const double xv = f >= 0 ? pv : 0.0;
+      s0 += A[(k * 9 + 0 * 3 + q) * nn + nn_] * xv;
+      s1 += A[(k * 9 + 1 * 3 + q) * nn + nn_] * xv;
+      s2 += A[(k * 9 + 2 * 3 + q) * nn + nn_] * xv;
+    }
+    tk[k][0][lane] = s0;
+    tk[k][1][lane] = s1;
+    tk[k][2][lane] = s2;
+  }
+  __syncthreads();
+  double a[1] = {0.0};
+  if (wv == 0 && valid) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int o = map_idx<OFF>(map, 3 * n + q);
+      if (o < 0) continue;
+      double acc = 0;
+#pragma unroll
+      for (int kk = 0; kk < 27; ++kk) acc += tk[kk][q][lane];
+      const double pv = FIRST ? __dmul_rn(dinv[o], r[o]) : pcg_dir(dinv[o], r[o], beta, pold[o]);
+      pnew[o] = pv;
+      Ap[o] = acc;
+      a[0] += pv * acc;
+    }
+  }
+  finish_reduce<1>(a, partials, ticket, s + 2, 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_update(long n, const double* __restrict__ dinv,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ Ap,
+                                                       double* __restrict__ x,
+                                                       double* __restrict__ r, double* partials,
+                                                       int* ticket, double* s, int cur,
+                                                       const int* gate) {
+  if (gate && *gate == 0) return;
+  const double alpha = s[cur] / (s[2] + 1e-300);
+  double a[1] = {0.0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    a[0] += ri * __dmul_rn(dinv[i], ri);
+  }
+  finish_reduce<1>(a, partials, ticket, s + (cur ^ 1), 0);
+}
+
+int fixed_pcg_grid(Work& w, const Grid& g, const DofMap& map, const double* A,
+                   const double* dinv, int n_free, int iters, const double* b, double* x,
+                   double* work, const int* gate, hipStream_t st) {
+  const long n = n_free;
+  double* r = work;
+  double* pb[2] = {work + n, work + 2 * n};
+  double* Ap = work + 3 * n;
+  double* s = work + 4 * n;
+  const int G = red_grid(n);
+  const int Gs = cdiv(g.n_nodes, 64);
+  if (Gs > kRedBlocks) {
+    set_error("fixed_pcg_grid: coarsest grid too large for one reduction grid");
+    return -2;
+  }
+  hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(kBlock), 0, st, n, b, dinv, x, r, w.partials,
+                     w.ticket, s, gate);
   GF_CHECK_LAUNCH();
+  for (int it = 0; it < iters; ++it) {
+    // r.z of this iteration sits in s[cur]; the previous one in s[cur ^ 1]
+    const int cur = it & 1;
+    double* pn = pb[cur];
+    const double* po = pb[cur ^ 1];
+#define GF_PCG_SPMV(OFF, FIRST)                                                                \
+  hipLaunchKernelGGL((k_pcg_spmv<OFF, FIRST>), dim3(Gs), dim3(kBlock), 0, st, g, map, A, dinv, \
+                     r, po, pn, Ap, w.partials, w.ticket, s, cur, gate)
+    if (map.off >= 0) {
+      if (it == 0) GF_PCG_SPMV(true, true); else GF_PCG_SPMV(true, false);
+    } else {
+      if (it == 0) GF_PCG_SPMV(false, true); else GF_PCG_SPMV(false, false);
+    }
+#undef GF_PCG_SPMV
+    GF_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(kBlock), 0, st, n, dinv, pn, Ap, x, r,
+                       w.partials, w.ticket, s, cur, gate);
+    GF_CHECK_LAUNCH();
+  }
   return 0;
 }
 

@@ -23,6 +23,11 @@ struct Work {
 // by the partials, two slots per block).
 constexpr int kSweepMaxBlocks = 1024;
 
+// Fine-operator launch shapes timed by the autotuner, in this order: 1 lane
+// per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16, register-blocked
+// LDS tiles 82 (8 lanes x 2 nodes along z), 83 (8 x 3) and 42 (4 x 2).
+constexpr int kFineShapes = 7;
+
 // Epilogue applied by the operator kernels to each output DOF i:
 //   PLAIN : y_i = (A x)_i
 //   RESID : y_i = b_i - (A x)_i
@@ -128,9 +133,11 @@ int dense_inverse_from_chol(int n, const double* L, double* Y, double* Yt, doubl
                             hipStream_t st);
 int dense_gemv(int n, const double* K, const double* b, double* x, const int* gate,
                hipStream_t st);
-int fixed_pcg(const Grid& g, const DofMap& map, const double* A, const double* dinv,
-              int n_free, int iters, const double* b, double* x, double* work,
-              const int* gate, hipStream_t st);
+// Fixed-count Jacobi PCG (zero start) on a stencil level, one launch per
+// vector phase over the whole grid; work: 4 * n_free + 4 doubles.
+int fixed_pcg_grid(Work& w, const Grid& g, const DofMap& map, const double* A,
+                   const double* dinv, int n_free, int iters, const double* b, double* x,
+                   double* work, const int* gate, hipStream_t st);
 int f64_to_f32_block(long n, const double* A, float* A32, hipStream_t st);
 
 }  // namespace gf

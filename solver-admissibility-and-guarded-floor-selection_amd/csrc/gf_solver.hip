@@ -371,10 +371,10 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
   hipEvent_t e0, e1;
   GF_HIP(hipEventCreate(&e0));
   GF_HIP(hipEventCreate(&e1));
-  const int shapes[4] = {1, 4, 8, 16};
+  const int shapes[kFineShapes] = {1, 4, 8, 16, 82, 83, 42};
   float best = 1e30f;
   int best_shape = 4;
-  for (int si = 0; si < 4; ++si) {
+  for (int si = 0; si < kFineShapes; ++si) {
     const int s = shapes[si];
     Grid gg = g;
     gg.shape = s;
@@ -885,7 +885,8 @@ int Gmg::solve_coarsest(const double* rhs, double* out, const int* gate, hipStre
   } else {
     int it = p.coarse_pcg_iters;
     if (high) it = std::max(p.coarse_pcg_iters, 200);
-    GF_TRY(fixed_pcg(c.g, c.map(), c.A, c.dinv, c.n_free, it, rhs, dst, pcg_work, gate, st));
+    GF_TRY(fixed_pcg_grid(work, c.g, c.map(), c.A, c.dinv, c.n_free, it, rhs, dst, pcg_work,
+                          gate, st));
   }
   *result = dst;
   return 0;
@@ -1327,7 +1328,7 @@ int Gmg::setup(const double* E, const double* fine_diag, hipStream_t st) {
       // Y = L^-1 in CHy, its transpose in CHt (the triangular path is unused here)
       if (p.coarsest_solve == 0) GF_TRY(dense_inverse_from_chol(n, CH, CHy, CHt, CHi, st));
     }
-    if (chol_mode == 2 && !pcg_work) GF_TRY(dalloc(&pcg_work, 3L * c.n_free));
+    if (chol_mode == 2 && !pcg_work) GF_TRY(dalloc(&pcg_work, 4L * c.n_free + 4));
   }
   GF_HIP(hipStreamSynchronize(st));
   is_setup = true;

@@ -300,6 +300,67 @@ def label_states(states, *, workers: int = 12, load: str = "edge", densities=Non
     return rows
 
 
+LABEL_FIELDS = ["index", "dims", "kind", "seed", "probability", "volfrac", "trigger",
+                "probe_r50", "probe_r100", "recommended_rho_min", "n_floor_attempts",
+                "solve_converged", "solve_iters", "recorded_policy_time_s"]
+
+
+def write_labels(path: str, states, rows) -> None:
+    """The plain policy's labels of ``states`` as CSV: the state descriptor
+    (from which the density is regenerated) and the policy outcome."""
+    import csv
+
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=LABEL_FIELDS)
+        w.writeheader()
+        for i, (s, r) in enumerate(zip(states, rows)):
+            w.writerow({
+                "index": i, "dims": "x".join(str(d) for d in s["dims"]), "kind": s["kind"],
+                "seed": s["seed"], "probability": s.get("probability", ""),
+                "volfrac": s.get("volfrac", ""), "trigger": r["trigger"],
+                "probe_r50": repr(float(r["probe_r50"])),
+                "probe_r100": repr(float(r["probe_r100"])),
+                "recommended_rho_min": repr(float(r["recommended_rho_min"])),
+                "n_floor_attempts": r["n_floor_attempts"],
+                "solve_converged": r["solve_converged"], "solve_iters": r["solve_iters"],
+                "recorded_policy_time_s": repr(float(r["recorded_policy_time_s"])),
+            })
+
+
+def read_labels(path: str):
+    """(states, rows) from a label CSV written by ``write_labels``; each row
+    carries ``dims``, ``kind``, ``probe_r50``, ``recommended_rho_min`` and the
+    policy columns (no pooled input: ``attach_pooled`` adds it)."""
+    import csv
+
+    states, rows = [], []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            dims = tuple(int(v) for v in r["dims"].split("x"))
+            st = {"dims": dims, "kind": r["kind"], "seed": int(r["seed"])}
+            if r["kind"] == "bernoulli":
+                st["probability"] = float(r["probability"])
+            else:
+                st["volfrac"] = float(r["volfrac"])
+            states.append(st)
+            rows.append({"dims": dims, "kind": r["kind"], "trigger": r["trigger"],
+                         "probe_r50": float(r["probe_r50"]),
+                         "probe_r100": float(r["probe_r100"]),
+                         "recommended_rho_min": float(r["recommended_rho_min"]),
+                         "n_floor_attempts": int(r["n_floor_attempts"]),
+                         "solve_converged": int(r["solve_converged"]),
+                         "solve_iters": int(r["solve_iters"]),
+                         "recorded_policy_time_s": float(r["recorded_policy_time_s"])})
+    return states, rows
+
+
+def attach_pooled(states, rows) -> None:
+    """Adds the pooled network input ``X`` (host float32 [16, 8, 8]) to each
+    row, regenerating the density from the state descriptor."""
+    for s, r in zip(states, rows):
+        r["X"] = pool_density(density_of(s), s["dims"]).cpu().numpy()
+
+
 def training_tensors(rows):
     """(X, feat, y_reg, y_cls) device tensors from labelled rows."""
     import torch

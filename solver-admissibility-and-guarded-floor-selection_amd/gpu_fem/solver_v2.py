@@ -82,6 +82,11 @@ class FineOperator:
         return self.mf_op.matvec(x, self.E)
 
 
+# fine-operator launch shapes in the autotuner's timing order: (name, code)
+LAUNCH_SHAPES = (("node", 1), ("wave4", 4), ("tile8x8", 8), ("tile4x16", 16),
+                 ("zblk8x2", 82), ("zblk8x3", 83), ("zblk4x2", 42))
+
+
 class MatrixFreeKff:
     """y_free = K_ff(E) u_free without assembling K (solver_v2.py:1121-1251)."""
 
@@ -111,17 +116,18 @@ class MatrixFreeKff:
     def autotune(self):
         """Pick the fastest fine-operator launch shape for this grid (results
         are bitwise identical for every shape)."""
-        us = (C.c_float * 4)()
+        us = (C.c_float * len(LAUNCH_SHAPES))()
         shape = C.c_int32()
         _lib.call("gf_ctx_autotune", self._ctx.h, C.cast(us, C.c_void_p), C.byref(shape),
                   _lib.stream_handle())
         self.launch_shape = int(shape.value)
-        self.launch_timings_us = {"node": float(us[0]), "wave4": float(us[1]),
-                                  "tile8x8": float(us[2]), "tile4x16": float(us[3])}
+        self.launch_timings_us = {name: float(us[i])
+                                  for i, (name, _) in enumerate(LAUNCH_SHAPES)}
         return self.launch_shape
 
     def set_launch_shape(self, shape: int):
-        """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles)."""
+        """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles,
+        82/83/42 register-blocked LDS tiles)."""
         _lib.call("gf_ctx_set_shape", self._ctx.h, int(shape))
         self.launch_shape = int(shape)
 

@@ -123,8 +123,12 @@ def test_hierarchy_setup_parity(dims, kind, rho_min):
                                    dict(root_enrichment_mode="weighted_gap_second"),
                                    dict(fine_smoother="fp32",
                                         level_precisions=["fp32", "fp32", "fp64", "fp64"]),
-                                   dict(coarsest_solve="triangular")])
+                                   dict(coarsest_solve="triangular"),
+                                   dict(dense_chol_max=40)])
 def test_preconditioner_parity(stack):
+    """One preconditioner application and one base cycle against the oracle.
+    dense_chol_max=40 sends the coarsest level (48 free DOFs here) through
+    the fixed-count Jacobi PCG path, as at 1M elements (6,825 > 5,000)."""
     ostack = {k: v for k, v in stack.items() if k != "coarsest_solve"}
     s = _setup((16, 8, 8), "tip", rho_min=1e-3, **ostack)
     if "coarsest_solve" in stack:
@@ -262,10 +266,12 @@ def test_coarsest_inverse_matches_triangular(dims, rho_min):
                                        ((12, 30, 6), "tip")])
 def test_fine_launch_shapes_bitwise_identical(dims, kind):
     """Every fine-operator launch shape (lane per node, four waves per 64
-    nodes, LDS tiles 8x8 and 4x16) gives the same bits, FP64 and FP32, with
+    nodes, LDS tiles 8x8 and 4x16, register-blocked LDS tiles with 2 or 3
+    z-nodes per lane) gives the same bits, FP64 and FP32, with
     the fused residual epilogue too (table- and offset-mode DOF maps, partial
     tiles)."""
     from gpu_fem import _lib
+    from gpu_fem.solver_v2 import LAUNCH_SHAPES
 
     s = _setup(dims, kind, rho_min=1e-3)
     mf_g = s["comps"][6]
@@ -273,18 +279,19 @@ def test_fine_launch_shapes_bitwise_identical(dims, kind):
     u = _t(rng.standard_normal(len(s["free"])))
     b = _t(rng.standard_normal(len(s["free"])))
     outs = {}
-    for shape in (1, 4, 8, 16):
+    shapes = [code for _, code in LAUNCH_SHAPES]
+    for shape in shapes:
         _lib.call("gf_ctx_set_shape", mf_g.ctx_handle, shape)
         outs[shape] = (_n(mf_g.matvec(u, s["Et"])),
                        _n(mf_g.matvec(u.float(), s["Et"].float())),
                        _n(mf_g.residual(b, u, s["Et"])))
-    for shape in (4, 8, 16):
+    for shape in shapes[1:]:
         for k in range(3):
             assert np.array_equal(outs[1][k], outs[shape][k]), (shape, k)
     y_o = s["mf"].matvec(_n(u), s["E"])
     assert np.abs(outs[8][0] - y_o).max() <= 1e-13 * np.abs(y_o).max()
     shape = mf_g.autotune()
-    assert shape in (1, 4, 8, 16)
+    assert shape in shapes
 
 
 def test_optimized_density_c64_golden(golden_dir):

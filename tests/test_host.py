@@ -187,3 +187,25 @@ def test_learned_compare_counts():
     assert c["floor_agreement"] == 0.5 and c["proposal_agreement"] == 0.5
     assert c["fallback_rate"] == 0.0 and c["max_accepted_true_rel_residual"] == 9e-7
     assert abs(c["r50_log10_rmse"] - np.sqrt(0.5)) < 1e-12
+
+
+def test_learned_label_csv_round_trip(tmp_path):
+    """Labels written by the labelling run read back as the same state
+    descriptors (so the densities regenerate) and the same policy outcome,
+    with floats exact."""
+    from gpu_fem.learned import density_of, read_labels, sample_states, write_labels
+
+    states = sample_states(4, 9, [(12, 6, 6), (10, 5, 5)])
+    rows = [{"trigger": "keep" if i % 2 else "high_r50", "probe_r50": 10.0 ** (-3 - i / 7),
+             "probe_r100": 0.1 / (i + 3), "recommended_rho_min": (1e-12, 1e-3)[i % 2],
+             "n_floor_attempts": 1 + i % 2, "solve_converged": 1, "solve_iters": 30 + i,
+             "recorded_policy_time_s": 0.123 * (i + 1)} for i in range(4)]
+    p = str(tmp_path / "labels.csv")
+    write_labels(p, states, rows)
+    st2, rows2 = read_labels(p)
+    assert st2 == states
+    for a, b in zip(rows, rows2):
+        for k, v in a.items():
+            assert b[k] == v, k
+    for a, b in zip(states, st2):
+        assert np.array_equal(density_of(a), density_of(b))
