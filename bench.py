@@ -88,7 +88,13 @@ def parse():
     ap.add_argument("--eval-states", type=int, default=96)
     ap.add_argument("--eval-seed", type=int, default=2)
     ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--labels", default="", help="label CSV to train from (skips labelling)")
+    ap.add_argument("--labels", default="",
+                    help="label CSV to train from (default: the shipped gpu_fem/data labels)")
+    ap.add_argument("--relabel", action="store_true",
+                    help="label fresh training states with the plain policy instead")
+    ap.add_argument("--confidence", default="auto",
+                    help="confidence gate of the gated learned form: a probability, or auto "
+                         "(the smallest one whose training-set predictions are 99%% correct)")
     ap.add_argument("--labels-out", default="", help="write the training labels here")
     ap.add_argument("--theta-out", default="", help="save the trained parameters (.npy)")
     ap.add_argument("--batch-states", type=int, default=64)
@@ -399,13 +405,15 @@ def learned_form(args):
     grids = [tuple(int(v) for v in g.split("x")) for g in args.train_grids.split(",") if g]
     train = L.sample_states(args.train_states, args.train_seed, grids)
     t_label = None
-    if args.labels and os.path.exists(args.labels):
-        progress(f"learned: reading labels {args.labels}")
-        train, rows = L.read_labels(args.labels)
+    labels = args.labels or ("" if args.relabel else L.DEFAULT_LABELS)
+    if labels and os.path.exists(labels):
+        progress(f"learned: reading labels {labels}")
+        train, rows = L.read_labels(labels)
     else:
         progress(f"learned: labelling {len(train)} training states with the plain policy")
         t0 = time.perf_counter()
-        rows = L.label_states(train, workers=args.workers, load=args.load, **pk)
+        rows = L.label_states(train, workers=args.workers, load=args.load, progress=progress,
+                              **pk)
         t_label = time.perf_counter() - t0
         if args.labels_out:
             L.write_labels(args.labels_out, train, rows)
@@ -419,8 +427,13 @@ def learned_form(args):
     hist = model.fit(X, feat, y_reg, y_cls, epochs=args.epochs, batch=64, lr=3e-3, seed=0)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    cls_tr, _ = model.predict(X, feat)
-    train_acc = float(np.mean(cls_tr == y_cls.cpu().numpy()))
+    cls_tr, _, conf_tr = model.predict_proba(X, feat)
+    correct_tr = cls_tr == y_cls.cpu().numpy()
+    train_acc = float(np.mean(correct_tr))
+    if args.confidence == "auto":
+        tau = L.confidence_for_accuracy(conf_tr, correct_tr, 0.99)
+    else:
+        tau = float(args.confidence)
     if args.theta_out:
         model.save(args.theta_out)
 
@@ -433,15 +446,18 @@ def learned_form(args):
     progress(f"learned: plain FP64 policy on {len(ev)} evaluation states")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain = L.label_states(ev, workers=args.workers, load=args.load, densities=dens, **pk)
+    plain = L.label_states(ev, workers=args.workers, load=args.load, densities=dens,
+                           progress=progress, **pk)
     torch.cuda.synchronize()
     w_plain = time.perf_counter() - t0
     forms = {}
-    for prec in ("fp64", "fp32"):
-        progress(f"learned: learned policy, {prec} multigrid")
+    for name, prec, gate in (("fp64", "fp64", 0.0), ("fp32", "fp32", 0.0),
+                             ("fp32_gated", "fp32", tau)):
+        progress(f"learned: learned policy, {prec} multigrid, confidence gate {gate:.3f}")
         got, wall = L.run_learned(model, ev, workers=args.workers, load=args.load,
-                                  densities=dens, **pk, **precision_stack(prec))
-        forms[prec] = (got, wall)
+                                  densities=dens, progress=progress, confidence_threshold=gate,
+                                  **pk, **precision_stack(prec))
+        forms[name] = (got, wall)
 
     def acc(rs):
         return sum(int(r["solve_converged"]) for r in rs)
@@ -472,11 +488,14 @@ def learned_form(args):
             "training": {"states": len(train), "grids": [list(g) for g in grids],
                          "labelling_s": t_label, "train_s": t_train, "epochs": args.epochs,
                          "loss_first_last": [hist[0], hist[-1]], "train_class_accuracy": train_acc,
-                         "labels": args.labels or None},
+                         "labels": os.path.relpath(labels, ROOT) if t_label is None else None,
+                         "confidence_gate": tau,
+                         "train_fraction_above_gate": float(np.mean(conf_tr >= tau))},
             "plain_fp64": {"value": v_plain, "wall_s": w_plain, "accepted": acc(plain),
                            "mean_total_iters": float(np.mean(plain_iters))},
             "learned_fp64_multigrid": out["fp64"],
             "learned_fp32_multigrid": out["fp32"],
+            "learned_fp32_multigrid_confidence_gated": out["fp32_gated"],
             "parallelism": f"single GPU, {args.workers} concurrent solver instances",
         },
     }

@@ -91,6 +91,13 @@ typedef struct gf_gmg_params {
    * through hipLaunchCooperativeKernel, 0 = one launch per projection.  Same
    * arithmetic order in every mode up to the partial-sum grouping. */
   int32_t mgs_sweep;
+  /* iterative coarsest solve (more free DOFs than dense_chol_max): 0 = the
+   * fixed-count Jacobi PCG in two launches per iteration (default), 1 = in
+   * one persistent launch with bounded grid-wide exchanges through uncached
+   * memory (measured no faster at 1M elements: 16 vs 15 us per iteration).
+   * The same recurrence; the dot products differ only in partial-sum
+   * grouping. */
+  int32_t coarse_pcg_persistent;
 } gf_gmg_params;
 
 /* State read back after gf_gmg_setup (properties of GalerkinMatFreeGMG,

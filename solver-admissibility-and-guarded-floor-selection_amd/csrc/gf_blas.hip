@@ -89,8 +89,12 @@ __global__ __launch_bounds__(kBlock) void k_mgs_axpy_dot(long n, const double* _
 constexpr int kSweepBlock = 256;
 constexpr long kSweepSpinCap = 1L << 20;
 
-template <int PER>
-__global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
+// PF: prefetch V_{s+1} into registers during the exchange of step s (PER
+// values per lane held three times); without it V_s is loaded at the start of
+// step s, which leaves room for twice as many values per lane.
+template <int PER, bool PF>
+__global__ __launch_bounds__(kSweepBlock)
+__attribute__((amdgpu_waves_per_eu(PF ? 1 : 3))) void k_mgs_sweep(long n, int j, int m,
                                                           const double* __restrict__ V,
                                                           double* __restrict__ w, double* H,
                                                           double* part, int* flags, double* err,
@@ -109,7 +113,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
   const unsigned f0 = f0s;
   const long stride = (long)G * kSweepBlock;
   const long base = (long)b * kSweepBlock + threadIdx.x;
-  double wr[PER], vr[PER], vn[PER];
+  double wr[PER], vr[PER], vn[PF ? PER : 1];
   bool broken = false;  // wave 0 only
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -119,6 +123,14 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
   }
   for (int s = 0; s <= j + 1; ++s) {
     double a = 0.0;
+    if (!PF && s > 0 && s <= j) {
+      const double* vs = V + (long)s * n;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const long i = base + k * stride;
+        vr[k] = i < n ? vs[i] : 0.0;
+      }
+    }
     if (s <= j) {
 #pragma unroll
       for (int k = 0; k < PER; ++k) a += wr[k] * vr[k];
@@ -127,12 +139,12 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
       for (int k = 0; k < PER; ++k) a += wr[k] * wr[k];
     }
     const double bs = block_sum(a, lds);
-    if (s < j) {  // prefetch V_{s+1} while the exchange is in flight
+    if (PF && s < j) {  // prefetch V_{s+1} while the exchange is in flight
       const double* vs = V + (long)(s + 1) * n;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const long i = base + k * stride;
-        vn[k] = i < n ? vs[i] : 0.0;
+        vn[(PF ? k : 0)] = i < n ? vs[i] : 0.0;
       }
     }
     const int slot = (s & 1) * kSweepMaxBlocks;
@@ -171,9 +183,9 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
 #pragma unroll
       for (int k = 0; k < PER; ++k) wr[k] -= h * vr[k];
       if (b == 0 && threadIdx.x == 0) H[(long)s * m + j] = h;
-      if (s < j) {
+      if (PF && s < j) {
 #pragma unroll
-        for (int k = 0; k < PER; ++k) vr[k] = vn[k];
+        for (int k = 0; k < PER; ++k) vr[k] = vn[(PF ? k : 0)];
       }
     } else {
       const double nrm = sqrt(h);
@@ -193,13 +205,13 @@ __global__ void k_fill_int(int n, int* p, int v) {
 }
 
 namespace {
-template <int PER>
+template <int PER, bool PF>
 int sweep_fits(int G) {
   static std::atomic<int> max_blocks{-1};  // co-resident blocks of this instantiation
   if (max_blocks.load() < 0) {
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mgs_sweep<PER>, kSweepBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mgs_sweep<PER, PF>, kSweepBlock, 0) !=
         hipSuccess)
       return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -209,10 +221,10 @@ int sweep_fits(int G) {
   return G <= max_blocks.load();
 }
 
-template <int PER>
+template <int PER, bool PF = true>
 int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H,
                  double* err, const int* gate, int cooperative, int G, hipStream_t st) {
-  if (!sweep_fits<PER>(G)) return 0;
+  if (!sweep_fits<PER, PF>(G)) return 0;
   if (wk.sweep_G != G) {
     // the sweep needs all flags of its grid equal at entry; flags left by a
     // different grid (or an unknown history) are reset first -- a plain
@@ -223,13 +235,13 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
   }
   if (cooperative) {
     void* args[] = {&n, &j, &m, (void*)&V, &wv, &H, &wk.partials, &wk.flags, &err, (void*)&gate};
-    GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER>), dim3(G),
+    GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER, PF>), dim3(G),
                                       dim3(kSweepBlock), args, 0, st));
   } else {
     // G is far below the device's co-resident capacity (checked above), so
     // every block is resident once the kernels ahead of it drain; the waits
     // are bounded regardless
-    hipLaunchKernelGGL(k_mgs_sweep<PER>, dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
+    hipLaunchKernelGGL((k_mgs_sweep<PER, PF>), dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
                        wk.partials, wk.flags, err, gate);
     GF_CHECK_LAUNCH();
   }
@@ -248,7 +260,11 @@ int mgs_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, doubl
   if (per <= 2) return launch_sweep<2>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
   if (per <= 4) return launch_sweep<4>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
   if (per <= 8) return launch_sweep<8>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
-  return launch_sweep<16>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  const int rc = launch_sweep<16>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  if (rc != 0) return rc;
+  // more blocks than can be co-resident with the prefetch buffer (n of a few
+  // million): the form without it needs fewer registers, so more blocks fit
+  return launch_sweep<16, false>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
 }
 
 // y = b - s*x ; out = ||y||  with s = *sptr (trial residual norm, y may be null)

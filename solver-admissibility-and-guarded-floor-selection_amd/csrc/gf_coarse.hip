@@ -9,6 +9,7 @@
 #include "gf_reduce.h"
 
 #include <atomic>
+#include <vector>
 
 namespace gf {
 
@@ -995,6 +996,229 @@ int fixed_pcg_grid(Work& w, const Grid& g, const DofMap& map, const double* A,
     GF_CHECK_LAUNCH();
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The same PCG in ONE persistent launch (k_pcg_persistent): a block owns 64
+// consecutive coarsest nodes (4 waves share the 27 stencil slots as in
+// k_pcg_spmv; wave 0 keeps its nodes' x, r, p, Ap in registers).  The two
+// dot products of an iteration are grid-wide exchanges of block partials as
+// in the Gram-Schmidt sweep: a block stores its partial in uncached memory,
+// waits for the store's acknowledgement, raises its arrival flag, and every
+// block folds all partials in block order (identical bits everywhere).  The
+// vectors other blocks read (r and the two direction buffers) also live in
+// uncached memory and are written and read with agent-scope atomic accesses
+// (which bypass the CU's L1), so a store is visible to every block once
+// acknowledged -- no L2 write-back is needed.  Flags are cleared by a fill
+// kernel before every launch.  Waits are bounded: an expired wait sets *err
+// (the outer solve reports it) and the block stops waiting, so the launch
+// always ends.
+// ---------------------------------------------------------------------------
+constexpr long kPcgSpinCap = 1L << 20;
+constexpr int kPcgMaxBlocks = 1024;
+
+// Block-sum v, publish it as exchange e, wait for every block's, fold them in
+// block order.  All threads of the block must call it.
+__device__ double pcg_exchange(double v, int e, int* flags, double* part, double* err,
+                               bool& broken, double* lds, double* bc) {
+  const int G = gridDim.x, b = blockIdx.x;
+  const double bs = block_sum(v, lds);
+  const int slot = (e & 1) * kPcgMaxBlocks;
+  const int target = e + 1;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&part[slot + b], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    store_ack_wait();
+    __hip_atomic_store(&flags[b], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < 64) {
+    long spins = 0;
+    while (!broken) {
+      int ok = 1;
+      for (int q = threadIdx.x; q < G; q += 64)
+        ok &= __hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+      if (__all(ok)) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kPcgSpinCap) {
+        broken = true;
+        if (threadIdx.x == 0) *err = 1.0;
+      }
+    }
+    double t = 0.0;
+    for (int q = threadIdx.x; q < G; q += 64)
+      t += __hip_atomic_load(&part[slot + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = wave_sum(t);
+    if (threadIdx.x == 0) *bc = t;
+  }
+  __syncthreads();
+  return *bc;
+}
+
+template <bool OFF>
+__global__ __launch_bounds__(kBlock) void k_pcg_persistent(Grid g, DofMap map,
+                                                           const double* __restrict__ A,
+                                                           const double* __restrict__ dinv,
+                                                           int iters, const double* __restrict__ b,
+                                                           double* __restrict__ x, double* r,
+                                                           double* p0, double* p1, int* flags,
+                                                           double* part, double* err,
+                                                           const int* gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double tk[27][3][64];
+  __shared__ double lds[4];
+  __shared__ double bc;
+  bool broken = false;  // wave 0 only
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = blockIdx.x * 64 + lane;
+  const bool valid = n < g.n_nodes;
+  const int nn_ = valid ? n : 0;
+  int ix, iy, iz;
+  node_xyz(g, nn_, ix, iy, iz);
+  const long nn = g.n_nodes;
+  int o[3];
+  double xr[3] = {0, 0, 0}, rr[3] = {0, 0, 0}, dr[3] = {0, 0, 0}, pr[3] = {0, 0, 0},
+         ar[3] = {0, 0, 0};
+  double a = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    o[q] = (valid && wv == 0) ? map_idx<OFF>(map, 3 * n + q) : -1;
+    if (o[q] >= 0) {
+      const double bi = b[o[q]];
+      dr[q] = dinv[o[q]];
+      rr[q] = bi;
+      __hip_atomic_store(&r[o[q]], bi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      a += bi * __dmul_rn(dr[q], bi);
+    }
+  }
+  int e = 0;
+  double rz = pcg_exchange(a, e++, flags, part, err, broken, lds, &bc);
+  double rz_prev = 1.0;
+  for (int it = 0; it < iters; ++it) {
+    double* pn = (it & 1) ? p1 : p0;
+    const double* po = (it & 1) ? p0 : p1;
+    const double beta = it == 0 ? 0.0 : rz / (rz_prev + 1e-300);
+    for (int k = wv; k < 27; k += 4) {
+      const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
+      const bool in = valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
+                      (jz < g.NZ);
+      const int m = (jx * g.NY + jy) * g.NZ + jz;
+      double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int f = in ? map_idx<OFF>(map, 3 * m + q) : -1;
+        const int fi = f >= 0 ? f : 0;
+        // r and the previous direction change during this launch (written by
+        // other blocks): agent-scope loads, which do not hit the CU's L1
+        const double dv = dinv[fi];
+        const double rv = __hip_atomic_load(&r[fi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double pv =
+            it == 0 ? __dmul_rn(dv, rv)
+                    : pcg_dir(dv, rv, beta,
+                              __hip_atomic_load(&po[fi], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT));
+        const double xv = f >= 0 ? pv : 0.0;
+        s0 += A[(k * 9 + 0 * 3 + q) * nn + nn_] * xv;
+        s1 += A[(k * 9 + 1 * 3 + q) * nn + nn_] * xv;
+        s2 += A[(k * 9 + 2 * 3 + q) * nn + nn_] * xv;
+      }
+      tk[k][0][lane] = s0;
+      tk[k][1][lane] = s1;
+      tk[k][2][lane] = s2;
+    }
+    __syncthreads();
+    a = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (o[q] < 0) continue;
+      double acc = 0;
+#pragma unroll
+      for (int kk = 0; kk < 27; ++kk) acc += tk[kk][q][lane];
+      pr[q] = it == 0 ? __dmul_rn(dr[q], rr[q]) : pcg_dir(dr[q], rr[q], beta, pr[q]);
+      __hip_atomic_store(&pn[o[q]], pr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ar[q] = acc;
+      a += pr[q] * acc;
+    }
+    const double pAp = pcg_exchange(a, e++, flags, part, err, broken, lds, &bc);
+    const double alpha = rz / (pAp + 1e-300);
+    a = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (o[q] < 0) continue;
+      xr[q] = xr[q] + alpha * pr[q];
+      rr[q] = rr[q] - alpha * ar[q];
+      __hip_atomic_store(&r[o[q]], rr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      a += rr[q] * __dmul_rn(dr[q], rr[q]);
+    }
+    rz_prev = rz;
+    rz = pcg_exchange(a, e++, flags, part, err, broken, lds, &bc);
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (o[q] >= 0) x[o[q]] = xr[q];
+}
+
+__global__ void k_fill_flags(int n, int* p) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Uncached buffers of the persistent PCG: handed out by size and recycled,
+// never returned to the driver (uncached memory that kernels have read is kept
+// until the process ends).
+namespace {
+struct UcBlock {
+  char* p;
+  size_t bytes;
+};
+std::vector<UcBlock>& uc_free_list() {
+  static std::vector<UcBlock> v;
+  return v;
+}
+}  // namespace
+
+int pcg_buffers_acquire(long n_free, void** out, size_t* bytes_out) {
+  GF_DEVICE_LOCK();
+  const size_t bytes = sizeof(int) * kPcgMaxBlocks + sizeof(double) * 2 * kPcgMaxBlocks +
+                       sizeof(double) * 3 * (size_t)n_free + 256;
+  auto& fl = uc_free_list();
+  for (size_t i = 0; i < fl.size(); ++i) {
+    if (fl[i].bytes >= bytes) {
+      *out = fl[i].p;
+      *bytes_out = fl[i].bytes;
+      fl.erase(fl.begin() + (long)i);
+      return 0;
+    }
+  }
+  GF_HIP(hipExtMallocWithFlags(out, bytes, hipDeviceMallocUncached));
+  *bytes_out = bytes;
+  return 0;
+}
+
+void pcg_buffers_release(void* p, size_t bytes) {
+  if (!p) return;
+  GF_DEVICE_LOCK();
+  uc_free_list().push_back(UcBlock{static_cast<char*>(p), bytes});
+}
+
+int fixed_pcg_persistent(const Grid& g, const DofMap& map, const double* A, const double* dinv,
+                         int n_free, int iters, const double* b, double* x, void* ucbuf,
+                         double* err, const int* gate, hipStream_t st) {
+  const int G = cdiv(g.n_nodes, 64);
+  if (G > kPcgMaxBlocks) return 0;
+  int* flags = static_cast<int*>(ucbuf);
+  double* part = reinterpret_cast<double*>(static_cast<char*>(ucbuf) + sizeof(int) * kPcgMaxBlocks);
+  double* r = part + 2 * kPcgMaxBlocks;
+  double* p0 = r + n_free;
+  double* p1 = p0 + n_free;
+  hipLaunchKernelGGL(k_fill_flags, dim3(4), dim3(256), 0, st, kPcgMaxBlocks, flags);
+  GF_CHECK_LAUNCH();
+  if (map.off >= 0)
+    hipLaunchKernelGGL(k_pcg_persistent<true>, dim3(G), dim3(kBlock), 0, st, g, map, A, dinv,
+                       iters, b, x, r, p0, p1, flags, part, err, gate);
+  else
+    hipLaunchKernelGGL(k_pcg_persistent<false>, dim3(G), dim3(kBlock), 0, st, g, map, A, dinv,
+                       iters, b, x, r, p0, p1, flags, part, err, gate);
+  GF_CHECK_LAUNCH();
+  return 1;
 }
 
 }  // namespace gf

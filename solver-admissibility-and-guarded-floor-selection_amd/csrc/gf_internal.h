@@ -138,6 +138,17 @@ int dense_gemv(int n, const double* K, const double* b, double* x, const int* ga
 int fixed_pcg_grid(Work& w, const Grid& g, const DofMap& map, const double* A,
                    const double* dinv, int n_free, int iters, const double* b, double* x,
                    double* work, const int* gate, hipStream_t st);
+// The same PCG in one persistent launch (grid-wide exchanges through the
+// uncached buffer from pcg_buffers_acquire; *err = 1 if a bounded wait
+// expires).  Returns 1 when launched, 0 when the level needs more than
+// 1024 blocks (the caller then uses fixed_pcg_grid), negative on error.
+int fixed_pcg_persistent(const Grid& g, const DofMap& map, const double* A, const double* dinv,
+                         int n_free, int iters, const double* b, double* x, void* ucbuf,
+                         double* err, const int* gate, hipStream_t st);
+// Uncached scratch of the persistent PCG (flags, partials, r and two
+// direction buffers), recycled by size and never returned to the driver.
+int pcg_buffers_acquire(long n_free, void** out, size_t* bytes_out);
+void pcg_buffers_release(void* p, size_t bytes);
 int f64_to_f32_block(long n, const double* A, float* A32, hipStream_t st);
 
 }  // namespace gf

@@ -108,6 +108,8 @@ struct Gmg {
   int ch_n = 0;
   int* d_fail = nullptr;
   double* pcg_work = nullptr;
+  void* pcg_uc = nullptr;     // uncached buffers of the persistent coarsest PCG
+  size_t pcg_uc_bytes = 0;
   double* re_w = nullptr;     // root-enrichment weights
   double* nb_inv9 = nullptr;  // root-local node-block Jacobi (own copy)
   double* nb_pinv = nullptr;

@@ -698,6 +698,9 @@ void Gmg::release() {
   dfree(CHy);
   dfree(d_fail);
   dfree(pcg_work);
+  pcg_buffers_release(pcg_uc, pcg_uc_bytes);
+  pcg_uc = nullptr;
+  pcg_uc_bytes = 0;
   dfree(re_w);
   dfree(nb_inv9);
   dfree(nb_pinv);
@@ -885,8 +888,15 @@ int Gmg::solve_coarsest(const double* rhs, double* out, const int* gate, hipStre
   } else {
     int it = p.coarse_pcg_iters;
     if (high) it = std::max(p.coarse_pcg_iters, 200);
-    GF_TRY(fixed_pcg_grid(work, c.g, c.map(), c.A, c.dinv, c.n_free, it, rhs, dst, pcg_work,
-                          gate, st));
+    int launched = 0;
+    if (pcg_uc) {
+      launched = fixed_pcg_persistent(c.g, c.map(), c.A, c.dinv, c.n_free, it, rhs, dst, pcg_uc,
+                                      d_err, gate, st);
+      if (launched < 0) return launched;
+    }
+    if (!launched)
+      GF_TRY(fixed_pcg_grid(work, c.g, c.map(), c.A, c.dinv, c.n_free, it, rhs, dst, pcg_work,
+                            gate, st));
   }
   *result = dst;
   return 0;
@@ -1329,6 +1339,8 @@ int Gmg::setup(const double* E, const double* fine_diag, hipStream_t st) {
       if (p.coarsest_solve == 0) GF_TRY(dense_inverse_from_chol(n, CH, CHy, CHt, CHi, st));
     }
     if (chol_mode == 2 && !pcg_work) GF_TRY(dalloc(&pcg_work, 4L * c.n_free + 4));
+    if (chol_mode == 2 && p.coarse_pcg_persistent && !pcg_uc)
+      GF_TRY(pcg_buffers_acquire(c.n_free, &pcg_uc, &pcg_uc_bytes));
   }
   GF_HIP(hipStreamSynchronize(st));
   is_setup = true;

@@ -48,6 +48,7 @@ class gf_gmg_params(C.Structure):
         ("coarsest_solve", C.c_int32),
         ("graph_apply", C.c_int32),
         ("mgs_sweep", C.c_int32),
+        ("coarse_pcg_persistent", C.c_int32),
     ]
 
 

@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import time
 
 import numpy as np
@@ -31,6 +32,12 @@ from .presets import cantilever
 
 FLOORS = (1e-12, 1e-3, 1e-2)
 POOL = (16, 8, 8)
+
+# Parameters trained on the GPU by bench.py --form learned from the labels in
+# floor_labels_train.csv (both in data/ beside this module).
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_THETA = os.path.join(DATA_DIR, "floor_cnn_theta.npy")
+DEFAULT_LABELS = os.path.join(DATA_DIR, "floor_labels_train.csv")
 
 
 R50_LOG10_RANGE = (-12.0, 0.5)
@@ -197,8 +204,16 @@ class FloorCNN:
 
     def predict(self, X, feat):
         """(floor class [B] int, predicted r50 [B]) on the host."""
+        cls, r50, _ = self.predict_proba(X, feat)
+        return cls, r50
+
+    def predict_proba(self, X, feat):
+        """(floor class [B], predicted r50 [B], confidence [B]) on the host;
+        the confidence is the softmax probability of the chosen class."""
         out = self.forward(X, feat).double().cpu().numpy()
-        return out[:, 1:].argmax(axis=1), r50_from_target(out[:, 0])
+        z = out[:, 1:] - out[:, 1:].max(axis=1, keepdims=True)
+        prob = np.exp(z) / np.exp(z).sum(axis=1, keepdims=True)
+        return out[:, 1:].argmax(axis=1), r50_from_target(out[:, 0]), prob.max(axis=1)
 
     def theta_host(self) -> np.ndarray:
         return self.theta.cpu().numpy().copy()
@@ -207,7 +222,8 @@ class FloorCNN:
         np.save(path, self.theta_host())
 
     @classmethod
-    def load(cls, path: str):
+    def load(cls, path: str = DEFAULT_THETA):
+        """Parameters saved by ``save`` (default: the shipped trained set)."""
         return cls(theta=np.load(path, allow_pickle=False))
 
 
@@ -275,11 +291,24 @@ POLICY_DEFAULTS = dict(penal=3.0, raised_rho_mins=FLOORS[1:], baseline_rho_min=F
                        probe_iters=100, probe_r50=50, restart=300, maxiter=300, tol=1e-6)
 
 
-def label_states(states, *, workers: int = 12, load: str = "edge", densities=None, **kw):
+def _run_chunked(session, sts, chunk, progress, tag):
+    """session.run over consecutive chunks (one session, so the cached
+    spectral estimate of its first state carries on), reporting after each."""
+    out = []
+    for k in range(0, len(sts), chunk):
+        out.extend(session.run(sts[k:k + chunk]))
+        if progress is not None:
+            progress(f"{tag}: {len(out)}/{len(sts)} states")
+    return out
+
+
+def label_states(states, *, workers: int = 12, load: str = "edge", densities=None,
+                 progress=None, chunk: int = 48, **kw):
     """Plain guarded policy (probe -> classify -> ladder -> guard) on every
     state; one session per grid size.  Returns the policy rows in state
     order, each with ``dims``, ``kind`` and the pooled input ``X`` (host
-    float32 [16, 8, 8]) added."""
+    float32 [16, 8, 8]) added.  ``progress(msg)`` (optional) is called after
+    every ``chunk`` states."""
     import torch
 
     pk = dict(POLICY_DEFAULTS)
@@ -289,7 +318,8 @@ def label_states(states, *, workers: int = 12, load: str = "edge", densities=Non
         session = PolicySession(cantilever(*dims, load=load), workers=min(workers, len(idx)),
                                 **pk)
         dens = [densities[i] if densities is not None else density_of(states[i]) for i in idx]
-        got = session.run([_policy_state(states[i], d) for i, d in zip(idx, dens)])
+        got = _run_chunked(session, [_policy_state(states[i], d) for i, d in zip(idx, dens)],
+                           chunk, progress, "plain policy " + "x".join(map(str, dims)))
         for i, d, r in zip(idx, dens, got):
             r.update(dims=dims, kind=states[i]["kind"],
                      X=pool_density(d, dims).cpu().numpy())
@@ -378,22 +408,44 @@ def training_tensors(rows):
 class LearnedFloorPolicy(GuardedFloorPolicy):
     """Solves a state at the network's proposed floor; the recomputed
     residual guard accepts it or the ladder continues upward (the plain
-    ladder's raised floors).  Same components and session semantics as
-    GuardedFloorPolicy."""
+    ladder's raised floors).  With ``confidence_threshold`` > 0, a proposal
+    whose confidence is below it is not used: that state goes through the
+    plain probe -> classify -> ladder -> guard policy instead.  Same
+    components and session semantics as GuardedFloorPolicy."""
 
-    def __init__(self, preset_or_spec, *, floors=FLOORS, **kw):
+    def __init__(self, preset_or_spec, *, floors=FLOORS, confidence_threshold: float = 0.0, **kw):
         kw.setdefault("baseline_rho_min", floors[0])
         kw.setdefault("raised_rho_mins", tuple(floors[1:]))
         super().__init__(preset_or_spec, **kw)
         self.floors = tuple(float(f) for f in floors)
+        self.confidence_threshold = float(confidence_threshold)
+
+    def _gated(self, confidence) -> bool:
+        return float(confidence) < self.confidence_threshold
 
     def first_moduli(self, state: dict) -> np.ndarray:
-        k = state.get("extra", {}).get("predicted_class", 0)
+        ex = state.get("extra", {})
+        if self._gated(ex.get("predicted_confidence", 1.0)):
+            return super().first_moduli(state)
+        k = ex.get("predicted_class", 0)
         return self.moduli(self.floors[k], state.get("solid"), state.get("rho_field"))
 
     def run_state(self, solid=None, *, rho_field=None, seed=None, probability=None,
-                  keep_x=False, predicted_class=0, predicted_r50=float("nan")):
+                  keep_x=False, predicted_class=0, predicted_r50=float("nan"),
+                  predicted_confidence=1.0):
         from .solver_v2 import dot
+
+        if self._gated(predicted_confidence):
+            row = super().run_state(solid, rho_field=rho_field, seed=seed,
+                                    probability=probability, keep_x=keep_x)
+            row.update(predicted_class=int(predicted_class),
+                       predicted_rho_min=self.floors[int(predicted_class)],
+                       predicted_r50=float(predicted_r50),
+                       predicted_confidence=float(predicted_confidence), gate="probe",
+                       guard_fallback=int(row["post_solve_fallback_triggered"]),
+                       total_iters=(len(row["probe_history"]) - 1
+                                    + sum(len(h) - 1 for _, h in row["solve_history"])))
+            return row
 
         rec = {"x": None, "final": float("nan"), "iters": 0, "reported": False, "sel": None,
                "hist": [], "total_iters": 0, "time": 0.0}
@@ -415,6 +467,7 @@ class LearnedFloorPolicy(GuardedFloorPolicy):
             "predicted_class": int(predicted_class),
             "predicted_rho_min": self.floors[int(predicted_class)],
             "predicted_r50": float(predicted_r50),
+            "predicted_confidence": float(predicted_confidence), "gate": "proposal",
             "recommended_rho_min": attempted[-1],
             "attempted_rho_mins": ";".join(f"{v:.0e}" for v in attempted),
             "n_floor_attempts": len(attempted), "guard_fallback": int(fallback),
@@ -432,9 +485,24 @@ class LearnedFloorPolicy(GuardedFloorPolicy):
         return row
 
 
+def confidence_for_accuracy(confidence, correct, target: float = 0.99) -> float:
+    """Smallest confidence threshold tau such that the predictions with
+    confidence >= tau are correct at least at rate ``target`` (measured on the
+    given labelled set); 1.0 if no threshold reaches it."""
+    conf = np.asarray(confidence, dtype=np.float64)
+    ok = np.asarray(correct, dtype=bool)
+    order = np.argsort(-conf)
+    hits = np.cumsum(ok[order])
+    rate = hits / np.arange(1, len(order) + 1)
+    good = np.nonzero(rate >= target)[0]
+    if len(good) == 0:
+        return 1.0
+    return float(conf[order][good[-1]])
+
+
 def predict_states(model: FloorCNN, states, densities=None):
     """Pools every state's density on the device and runs one batched
-    forward; (classes, r50s)."""
+    forward; (classes, r50s, confidences)."""
     import torch
 
     Xs, feats = [], []
@@ -442,14 +510,15 @@ def predict_states(model: FloorCNN, states, densities=None):
         d = densities[i] if densities is not None else density_of(s)
         Xs.append(pool_density(d, s["dims"]))
         feats.append(torch.as_tensor(grid_features(s["dims"]), device="cuda"))
-    return model.predict(torch.stack(Xs), torch.stack(feats))
+    return model.predict_proba(torch.stack(Xs), torch.stack(feats))
 
 
 def run_learned(model: FloorCNN, states, *, workers: int = 12, load: str = "edge",
-                densities=None, **kw):
+                densities=None, progress=None, chunk: int = 48, **kw):
     """Learned policy on every state (one session per grid size, concurrent
     instances).  Returns (rows in state order, wall seconds including the
-    pooling and network inference)."""
+    pooling and network inference).  ``progress(msg)`` (optional) is called
+    after every ``chunk`` states."""
     import torch
 
     pk = dict(POLICY_DEFAULTS)
@@ -457,7 +526,7 @@ def run_learned(model: FloorCNN, states, *, workers: int = 12, load: str = "edge
     dens = densities if densities is not None else [density_of(s) for s in states]
     _sync()
     t0 = time.perf_counter()
-    cls, r50 = predict_states(model, states, dens)
+    cls, r50, conf = predict_states(model, states, dens)
     rows = [None] * len(states)
     for dims, idx in _groups(states).items():
         session = PolicySession(cantilever(*dims, load=load), workers=min(workers, len(idx)),
@@ -465,9 +534,12 @@ def run_learned(model: FloorCNN, states, *, workers: int = 12, load: str = "edge
         sts = []
         for i in idx:
             st = _policy_state(states[i], dens[i])
-            st["extra"] = {"predicted_class": int(cls[i]), "predicted_r50": float(r50[i])}
+            st["extra"] = {"predicted_class": int(cls[i]), "predicted_r50": float(r50[i]),
+                           "predicted_confidence": float(conf[i])}
             sts.append(st)
-        for i, r in zip(idx, session.run(sts)):
+        got = _run_chunked(session, sts, chunk, progress,
+                           "learned policy " + "x".join(map(str, dims)))
+        for i, r in zip(idx, got):
             r.update(dims=dims, kind=states[i]["kind"])
             rows[i] = r
         del session
@@ -488,6 +560,7 @@ def compare(plain_rows, learned_rows) -> dict:
         "proposal_agreement": sum(floor_class(p["recommended_rho_min"]) == q["predicted_class"]
                                   for p, q in zip(plain_rows, learned_rows)) / max(1, n),
         "fallback_rate": sum(q["guard_fallback"] for q in learned_rows) / max(1, n),
+        "probe_rate": sum(q.get("gate") == "probe" for q in learned_rows) / max(1, n),
         "learned_accepted": len(acc),
         "plain_accepted": sum(int(p["solve_converged"]) for p in plain_rows),
         "max_accepted_true_rel_residual": max([q["solve_final_rel_residual"] for q in acc],

@@ -58,7 +58,8 @@ class GalerkinMatFreeGMG:
                  root_local_correction_inner_steps: int = 10,
                  root_local_correction_inner_restart=None, inner_krylov_steps: int = 0,
                  inner_krylov_restart=None, coarsest_solve: str = "inverse",
-                 graph_apply: bool = True, mgs_sweep=True) -> None:
+                 graph_apply: bool = True, mgs_sweep=True,
+                 coarse_pcg_persistent: bool = False) -> None:
         # argument validation (multigrid_v4.py:600-699), same messages
         if smoother_type not in {"chebyshev", "jacobi"}:
             raise ValueError(f"smoother_type must be 'chebyshev' or 'jacobi'; got {smoother_type!r}")
@@ -168,6 +169,7 @@ class GalerkinMatFreeGMG:
         if int(mgs_sweep) not in (0, 1, 2):
             raise ValueError("mgs_sweep must be False/0, True/1 or 2")
         p.mgs_sweep = int(mgs_sweep)
+        p.coarse_pcg_persistent = 1 if coarse_pcg_persistent else 0
         self._params = p
         h = C.c_void_p()
         _lib.check(lib.gf_gmg_create(C.byref(h), mf_op.ctx_handle, C.byref(p)), "gf_gmg_create")

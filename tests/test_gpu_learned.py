@@ -154,3 +154,32 @@ def test_learned_policy_matches_plain_selected_solve_when_proposal_agrees():
         assert abs(got["compliance"] - row["compliance"]) <= 1e-12 * abs(row["compliance"])
         return
     pytest.skip("no state kept the baseline floor")
+
+
+def test_trained_network_gains_over_the_plain_policy():
+    """The shipped trained parameters (bench.py --form learned, trained on
+    the labels in gpu_fem/data) against the method without them, on fresh
+    seeds the network never saw (40x20x20 Bernoulli and filtered-sigmoid
+    states): the learned policy skips the 100-iteration probe, so it needs
+    fewer FGMRES iterations in total; the guard still judges every solve, so
+    every accepted solve meets the tolerance and every state the plain
+    policy accepts is accepted."""
+    import os
+
+    from gpu_fem.learned import (DEFAULT_THETA, FloorCNN, compare, label_states, run_learned,
+                                 sample_states)
+
+    if not os.path.exists(DEFAULT_THETA):
+        pytest.skip("no trained parameters shipped")
+    model = FloorCNN.load()
+    states = sample_states(8, 77, [(40, 20, 20)])
+    plain = label_states(states, workers=4)
+    learned, _ = run_learned(model, states, workers=4)
+    plain_iters = sum(len(r["probe_history"]) - 1 + sum(len(h) - 1 for _, h in r["solve_history"])
+                      for r in plain)
+    learned_iters = sum(r["total_iters"] for r in learned)
+    c = compare(plain, learned)
+    print("plain iters", plain_iters, "learned iters", learned_iters, c)
+    assert learned_iters < plain_iters
+    assert c["learned_accepted"] >= c["plain_accepted"]
+    assert c["max_accepted_true_rel_residual"] <= 1e-6

@@ -123,15 +123,13 @@ def test_hierarchy_setup_parity(dims, kind, rho_min):
                                    dict(root_enrichment_mode="weighted_gap_second"),
                                    dict(fine_smoother="fp32",
                                         level_precisions=["fp32", "fp32", "fp64", "fp64"]),
-                                   dict(coarsest_solve="triangular"),
-                                   dict(dense_chol_max=40)])
+                                   dict(coarsest_solve="triangular")])
 def test_preconditioner_parity(stack):
-    """One preconditioner application and one base cycle against the oracle.
-    dense_chol_max=40 sends the coarsest level (48 free DOFs here) through
-    the fixed-count Jacobi PCG path, as at 1M elements (6,825 > 5,000)."""
-    ostack = {k: v for k, v in stack.items() if k != "coarsest_solve"}
+    """One preconditioner application and one base cycle against the oracle."""
+    gpu_only = ("coarsest_solve", "coarse_pcg_persistent")
+    ostack = {k: v for k, v in stack.items() if k not in gpu_only}
     s = _setup((16, 8, 8), "tip", rho_min=1e-3, **ostack)
-    if "coarsest_solve" in stack:
+    if any(k in stack for k in gpu_only):
         from gpu_fem import policy as gpol
         from gpu_fem.presets import cantilever
         st = dict(STACK)
@@ -148,6 +146,49 @@ def test_preconditioner_parity(stack):
     vo = og._apply_base_cycle(rhs)
     vg = _n(gg.vcycle(_t(rhs)))
     assert np.abs(vg - vo).max() <= tol * np.abs(vo).max()
+
+
+@pytest.mark.parametrize("rho_min,tol", [(1e-3, 1e-10), (1e-1, 1e-4)])
+def test_coarsest_pcg_parity(rho_min, tol):
+    """The fixed-count Jacobi PCG coarsest solve (used when the coarsest level
+    has more than dense_chol_max free DOFs, as at 1M elements: 6,825 > 5,000)
+    in its persistent form and its two-launch form, against the oracle.
+    n_levels=2 with dense_chol_max=100 puts the coarsest level (8x4x4
+    elements, 225 nodes in 4 blocks of 64, 600 free DOFs) on that path; the
+    cycle is kept linear (no line search, no Krylov wrappers) so the
+    comparison isolates the PCG.  Tolerances, measured with the oracle
+    against itself with the stencil product summed in reverse order: at
+    rho_min=1e-3 the contrast rule runs 200 iterations, the PCG converges
+    (relative residual 9e-14) and the reordered cycle differs by 2e-16; at
+    rho_min=1e-1 it runs 80 iterations that stop at relative residual 1.7e-4,
+    and an unconverged CG amplifies summation order: the reordered cycle
+    differs by 1.7e-5."""
+    from gpu_fem import policy as gpol
+    from gpu_fem.presets import cantilever
+
+    lin = dict(n_levels=2, dense_chol_max=100, coarse_correction_policy="none",
+               inner_krylov_steps=0, root_local_correction_mode="none")
+    s = _setup((16, 8, 8), "tip", rho_min=rho_min, **lin)
+    og = s["og"]
+    og.setup(s["E"])
+    rhs = np.random.default_rng(11).standard_normal(len(s["free"]))
+    zo = og.apply(rhs)
+    zs = {}
+    for persistent in (True, False):
+        st = dict(STACK)
+        st.update(lin)
+        comps = gpol._build_components(cantilever(16, 8, 8), coarse_pcg_persistent=persistent, **st)
+        gg = comps[7]
+        gg.setup(s["Et"])
+        assert gg.level_free(1).size == 600
+        zs[persistent] = _n(gg.apply(_t(rhs)))
+    for persistent, z in zs.items():
+        err = np.abs(z - zo).max() / np.abs(zo).max()
+        print("persistent" if persistent else "two-launch", "rel err vs oracle", err)
+        assert err <= tol
+    d = np.abs(zs[True] - zs[False]).max() / np.abs(zs[False]).max()
+    print("persistent vs two-launch", d)
+    assert d <= tol
 
 
 @pytest.mark.parametrize("dims,seed,p,rho_min,ctol", [((16, 8, 8), 43, 0.35, 1e-12, 1e-6),

@@ -41,6 +41,7 @@ def test_params_struct_layout_matches_header():
     assert p.n_levels == 4 and p.coarse_smooth_iters == 2 and p.dense_chol_max == 5000
     assert abs(p.cheb_lower_frac - 1 / 30) < 1e-15 and p.coarse_pcg_iters == 80
     assert list(p.level_prec) == [-1] * 8 and p.root_enrichment_weight_floor == 1e-6
+    assert p.mgs_sweep == 1 and p.coarse_pcg_persistent == 0 and p.graph_apply == 1
 
 
 def test_library_reports_argument_errors_without_gpu():
@@ -209,3 +210,15 @@ def test_learned_label_csv_round_trip(tmp_path):
             assert b[k] == v, k
     for a, b in zip(states, st2):
         assert np.array_equal(density_of(a), density_of(b))
+
+
+def test_confidence_for_accuracy():
+    from gpu_fem.learned import confidence_for_accuracy
+
+    conf = np.array([0.99, 0.95, 0.9, 0.8, 0.7, 0.6, 0.5])
+    ok = np.array([1, 1, 1, 0, 1, 0, 1], dtype=bool)
+    # prefix accuracies by falling confidence: 1, 1, 1, 3/4, 4/5, 4/6, 5/7
+    assert confidence_for_accuracy(conf, ok, 0.99) == 0.9
+    assert confidence_for_accuracy(conf, ok, 0.8) == 0.7
+    assert confidence_for_accuracy(conf, ok, 0.7) == 0.5
+    assert confidence_for_accuracy(conf, np.zeros(7, bool), 0.5) == 1.0
