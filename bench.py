@@ -38,9 +38,7 @@ FP64_PEAK_TFLOPS = 78.6
 # (profiles/fine_matvec_fp64_80x40x20_roofline.md)
 TRAFFIC_BYTES = 4309925
 SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
-                 8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>",
-                 82: "k_fine_matvec_zblk<TZ=8,R=2>", 83: "k_fine_matvec_zblk<TZ=8,R=3>",
-                 42: "k_fine_matvec_zblk<TZ=4,R=2>"}
+                 8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
 
 
 def progress(msg):
@@ -87,6 +85,13 @@ def parse():
     ap.add_argument("--train-seed", type=int, default=1)
     ap.add_argument("--eval-states", type=int, default=96)
     ap.add_argument("--eval-seed", type=int, default=2)
+    ap.add_argument("--eval-kinds", default="bernoulli,sigmoid")
+    ap.add_argument("--eval-volfrac", type=float, default=0.0,
+                    help="fix the filtered-sigmoid volume fraction of the evaluation states")
+    ap.add_argument("--eval-workers", type=int, default=0,
+                    help="concurrent instances of the evaluation (0: --workers)")
+    ap.add_argument("--learned-variants", default="fp64,fp32,fp32_gated")
+    ap.add_argument("--eval-warmup", type=int, default=1, choices=[0, 1])
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--labels", default="",
                     help="label CSV to train from (default: the shipped gpu_fem/data labels)")
@@ -389,12 +394,13 @@ def learned_form(args):
     densities on the --train-grids) are labelled by the plain FP64 guarded
     policy -- or read from --labels, a CSV an earlier labelling run wrote --
     and the 3D CNN is trained on the GPU by the library's own BF16 kernels.
-    Fresh-seed evaluation states on --grid then run three ways, each as
-    concurrent solver instances: the plain FP64 policy (probe + ladder +
-    guard), the learned policy with the FP64 multigrid, and the learned policy
-    with the FP32 multigrid.  ``value`` is the last one's accepted guarded
-    solves/s; the plain form's rate, the speed-ups, the floor agreement with
-    the plain ladder and the guard fallback rate are reported beside it."""
+    Fresh-seed evaluation states on --grid then run through the plain FP64
+    policy (probe + ladder + guard) on one solver instance and on concurrent
+    instances, and through the learned policy variants (FP64 or FP32
+    multigrid, optionally confidence-gated) on concurrent instances.
+    ``value`` is the learned FP32-multigrid variant's accepted guarded
+    solves/s; both plain rates, the speed-ups, the floor agreement with the
+    plain ladder and the guard fallback rate are reported beside it."""
     import numpy as np
     import torch
 
@@ -438,65 +444,87 @@ def learned_form(args):
         model.save(args.theta_out)
 
     dims = tuple(int(v) for v in args.grid.split("x"))
-    ev = L.sample_states(args.eval_states, args.eval_seed, [dims])
+    kinds = tuple(k for k in args.eval_kinds.split(",") if k)
+    vf = (args.eval_volfrac, args.eval_volfrac) if args.eval_volfrac else (0.2, 0.5)
+    ev = L.sample_states(args.eval_states, args.eval_seed, [dims], kinds=kinds, volfrac_range=vf)
     dens = [L.density_of(s) for s in ev]
-    # warm-up: kernels and pools of this grid (one state each way)
-    progress("learned: warm-up")
-    L.label_states(ev[:1], workers=1, load=args.load, densities=dens[:1], **pk)
-    progress(f"learned: plain FP64 policy on {len(ev)} evaluation states")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain = L.label_states(ev, workers=args.workers, load=args.load, densities=dens,
-                           progress=progress, **pk)
-    torch.cuda.synchronize()
-    w_plain = time.perf_counter() - t0
-    forms = {}
-    for name, prec, gate in (("fp64", "fp64", 0.0), ("fp32", "fp32", 0.0),
-                             ("fp32_gated", "fp32", tau)):
-        progress(f"learned: learned policy, {prec} multigrid, confidence gate {gate:.3f}")
-        got, wall = L.run_learned(model, ev, workers=args.workers, load=args.load,
-                                  densities=dens, progress=progress, confidence_threshold=gate,
-                                  **pk, **precision_stack(prec))
-        forms[name] = (got, wall)
+    workers = args.eval_workers or args.workers
+    if args.eval_warmup:  # kernels, pools and autotuning of this grid
+        progress("learned: warm-up")
+        L.label_states(ev[:1], workers=1, load=args.load, densities=dens[:1], **pk)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rows = fn()
+        torch.cuda.synchronize()
+        return rows, time.perf_counter() - t0
 
     def acc(rs):
         return sum(int(r["solve_converged"]) for r in rs)
 
+    progress(f"learned: plain FP64 policy, one instance, {len(ev)} evaluation states")
+    plain_seq, w_seq = timed(lambda: L.label_states(ev, workers=1, load=args.load,
+                                                    densities=dens, progress=progress, **pk))
+    progress(f"learned: plain FP64 policy, {workers} concurrent instances")
+    plain, w_plain = timed(lambda: L.label_states(ev, workers=workers, load=args.load,
+                                                  densities=dens, progress=progress, **pk))
+    variants = {"fp64": ("fp64", 0.0), "fp32": ("fp32", 0.0), "fp32_gated": ("fp32", tau)}
+    forms = {}
+    for name in (v for v in args.learned_variants.split(",") if v):
+        prec, gate = variants[name]
+        progress(f"learned: learned policy, {prec} multigrid, confidence gate {gate:.3f}")
+        got, wall = L.run_learned(model, ev, workers=workers, load=args.load, densities=dens,
+                                  progress=progress, confidence_threshold=gate, **pk,
+                                  **precision_stack(prec))
+        forms[name] = (got, wall)
+
+    v_seq = acc(plain_seq) / w_seq
     v_plain = acc(plain) / w_plain
     out = {}
-    for prec, (got, wall) in forms.items():
+    for name, (got, wall) in forms.items():
         c = L.compare(plain, got)
-        c.update(value=acc(got) / wall, wall_s=wall, speedup_vs_plain_fp64=(acc(got) / wall) / v_plain,
+        c.update(value=acc(got) / wall, wall_s=wall,
+                 speedup_vs_plain_fp64_one_instance=(acc(got) / wall) / v_seq,
+                 speedup_vs_plain_fp64_concurrent=(acc(got) / wall) / v_plain,
                  mean_total_iters=float(np.mean([r["total_iters"] for r in got])))
-        out[prec] = c
-    r32, w32 = forms["fp32"]
-    plain_iters = [len(r["probe_history"]) - 1 + sum(len(h) - 1 for _, h in r["solve_history"])
-                   for r in plain]
+        out[name] = c
+    head = "fp32" if "fp32" in out else next(iter(out))
+    w_head = forms[head][1]
+
+    def iters(r):
+        return len(r["probe_history"]) - 1 + sum(len(h) - 1 for _, h in r["solve_history"])
+
     line = {
         "metric": ("accepted guarded solves/s, learned floor + FP32 multigrid + concurrent "
                    "instances vs plain FP64"),
-        "value": out["fp32"]["value"], "unit": "accepted guarded solves/s", "n_gpus": 1,
-        "steps": len(ev), "warmup": 1, "ms_per_step": w32 / len(ev) * 1e3,
+        "value": out[head]["value"], "unit": "accepted guarded solves/s", "n_gpus": 1,
+        "steps": len(ev), "warmup": args.eval_warmup, "ms_per_step": w_head / len(ev) * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 outer / f32 preconditioner; network bf16 (fp32 accumulation)",
         "data": "synthetic (seeded Bernoulli and filtered-sigmoid density states)",
         "config": {
             "workload": (f"{len(ev)} fresh-seed states on cantilever {dims[0]}x{dims[1]}x{dims[2]} "
-                         f"(Bernoulli p in [0.1,0.5] and filtered-sigmoid volume fraction in "
-                         f"[0.2,0.5]), SIMP p={args.penal:g}, {args.load} load, ladder "
-                         "{1e-12,1e-3,1e-2}, restart 300, tol 1e-6"),
+                         f"({'/'.join(kinds)}; Bernoulli p in [0.1,0.5], filtered-sigmoid volume "
+                         f"fraction in [{vf[0]},{vf[1]}]), SIMP p={args.penal:g}, {args.load} load, "
+                         "ladder {1e-12,1e-3,1e-2}, restart 300, tol 1e-6"),
+            "headline_variant": head,
             "training": {"states": len(train), "grids": [list(g) for g in grids],
                          "labelling_s": t_label, "train_s": t_train, "epochs": args.epochs,
                          "loss_first_last": [hist[0], hist[-1]], "train_class_accuracy": train_acc,
                          "labels": os.path.relpath(labels, ROOT) if t_label is None else None,
                          "confidence_gate": tau,
                          "train_fraction_above_gate": float(np.mean(conf_tr >= tau))},
-            "plain_fp64": {"value": v_plain, "wall_s": w_plain, "accepted": acc(plain),
-                           "mean_total_iters": float(np.mean(plain_iters))},
-            "learned_fp64_multigrid": out["fp64"],
-            "learned_fp32_multigrid": out["fp32"],
-            "learned_fp32_multigrid_confidence_gated": out["fp32_gated"],
-            "parallelism": f"single GPU, {args.workers} concurrent solver instances",
+            "plain_fp64_one_instance": {"value": v_seq, "wall_s": w_seq,
+                                        "accepted": acc(plain_seq),
+                                        "mean_total_iters": float(np.mean([iters(r) for r in plain_seq]))},
+            "plain_fp64_concurrent": {"value": v_plain, "wall_s": w_plain, "accepted": acc(plain),
+                                      "mean_total_iters": float(np.mean([iters(r) for r in plain]))},
+            "plain_one_instance_equals_concurrent": all(
+                a["recommended_rho_min"] == b["recommended_rho_min"]
+                and a["solve_iters"] == b["solve_iters"] for a, b in zip(plain_seq, plain)),
+            "learned": out,
+            "parallelism": f"single GPU, {workers} concurrent solver instances",
         },
     }
     print(json.dumps(line))

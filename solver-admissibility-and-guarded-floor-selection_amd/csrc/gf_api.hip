@@ -58,9 +58,8 @@ int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* st
 
 int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape) {
   CHECK_PTR(ctx);
-  if (shape != 0 && shape != 1 && shape != 4 && shape != 8 && shape != 16 && shape != 82 &&
-      shape != 83 && shape != 42) {
-    set_error("gf_ctx_set_shape: shape must be 0, 1, 4, 8, 16, 82, 83 or 42");
+  if (shape != 0 && shape != 1 && shape != 4 && shape != 8 && shape != 16) {
+    set_error("gf_ctx_set_shape: shape must be 0, 1, 4, 8 or 16");
     return -2;
   }
   ctx->c.g.shape = shape;

@@ -318,148 +318,6 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
   if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
 }
 
-// Register-blocked form of the LDS-tiled kernel.  Each lane owns a run of R
-// consecutive z-nodes, so every Ke value (a wave-uniform scalar operand) and
-// every staged window value feeds R times as many FMAs as in the tile
-// kernel.  A block covers a TY x (TZ*R) (y,z) patch of one x-plane:
-//  * staging: the node window is 3 x (TY+2) rows of (TZ*R+2) consecutive
-//    nodes, i.e. 3*(TZ*R+2) consecutive DOFs per row; a wave takes whole rows
-//    (row index and its x/y bounds are wave-uniform) and its lanes read
-//    consecutive DOFs, coalesced, with no per-entry division but k / 3.  The
-//    element window is staged the same way (rows of TZ*R+1 elements).
-//  * compute: wave w forms the element pair (a,b) = (w>>1, w&1) for its R
-//    nodes with the same per-node expressions as the other shapes (row
-//    products in column order, P_ab = E_ab0*t_ab0 + E_ab1*t_ab1);
-//  * combine: every wave stores its pair sums, and all 256 threads form
-//    (P00+P01)+(P10+P11) for the patch's outputs in DOF order (coalesced
-//    epilogue), so the result is bitwise identical to the other shapes.
-template <typename T, int EPI, bool OFF, int TZ, int R>
-__global__ __launch_bounds__(kBlock) void k_fine_matvec_zblk(Grid g, DofMap map,
-                                                             const T* __restrict__ Ke,
-                                                             const T* __restrict__ E,
-                                                             const T* __restrict__ u,
-                                                             T* __restrict__ y, EpiArgs<T> ep,
-                                                             const int* gate) {
-  using A = typename Acc<T>::type;
-  constexpr int TY = 64 / TZ;
-  constexpr int PZ = TZ * R;                 // patch nodes along z
-  constexpr int WY = TY + 2, WZ = PZ + 2;    // node window (x: 3 planes)
-  constexpr int EY = TY + 1, EZ = PZ + 1;    // element window (x: 2 planes)
-  constexpr int ROW = 3 * WZ;                // DOFs per node-window row
-  constexpr int NU = 3 * WY * ROW, NE = 2 * EY * EZ;
-  constexpr int NOUT = 3 * PZ;               // outputs per y-row of the patch
-  if (gate && *gate == 0) return;
-  __shared__ T us[NU];
-  __shared__ T es[NE];
-  __shared__ A part[4][3 * R][64];
-  const int tiles_z = (g.NZ + PZ - 1) / PZ, tiles_y = (g.NY + TY - 1) / TY;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, ix = tile / (tiles_z * tiles_y);
-  const int y0 = ty * TY, z0 = tz * PZ;
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // node window row (xx, yy): nodes (ix-1+xx, y0-1+yy, z0-1 .. z0+PZ), entry
-  // (row*WZ + zz)*3 + q holds DOF q of node z0-1+zz
-  for (int row = wv; row < 3 * WY; row += 4) {
-    const int xx = row / WY, yy = row - xx * WY;
-    const int jx = ix - 1 + xx, jy = y0 - 1 + yy;
-    const bool in_xy = (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY);
-    const int dof0 = 3 * ((jx * g.NY + jy) * g.NZ + z0 - 1);
-#pragma unroll
-    for (int k0 = 0; k0 < ROW; k0 += 64) {
-      const int k = k0 + lane;
-      if (k < ROW) {
-        const int jz = z0 - 1 + k / 3;
-        const bool in = in_xy & (jz >= 0) & (jz < g.NZ);
-        us[row * ROW + k] = map_load<OFF>(map, u, dof0 + k, in);
-      }
-    }
-  }
-  // element window row (ex, ey): elements (ix-1+ex, y0-1+ey, z0-1 .. z0+PZ-1)
-  for (int row = wv; row < 2 * EY; row += 4) {
-    const int ex = row / EY, ey = row - ex * EY;
-    const int cx = ix - 1 + ex, cy = y0 - 1 + ey;
-    const bool in_xy = (cx >= 0) & (cx < g.nx) & (cy >= 0) & (cy < g.ny);
-    const int e0 = (cx * g.ny + cy) * g.nz + z0 - 1;
-#pragma unroll
-    for (int k0 = 0; k0 < EZ; k0 += 64) {
-      const int k = k0 + lane;
-      if (k < EZ) {
-        const int cz = z0 - 1 + k;
-        const bool in = in_xy & (cz >= 0) & (cz < g.nz);
-        const T ev = E[in ? e0 + k : 0];
-        es[row * EZ + k] = in ? ev : T(0);
-      }
-    }
-  }
-  __syncthreads();
-  const int a = wv >> 1, b = wv & 1;
-  const int ly = lane / TZ, lz = lane % TZ;
-  // w[px][py][k][q]: node (ix-1+a+px, y0+ly-1+b+py, z0+lz*R-1+k)
-  T w[2][2][R + 2][3];
-#pragma unroll
-  for (int px = 0; px < 2; ++px)
-#pragma unroll
-    for (int py = 0; py < 2; ++py)
-#pragma unroll
-      for (int k = 0; k < R + 2; ++k)
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-          w[px][py][k][q] = us[((a + px) * WY + (ly + b + py)) * ROW + (lz * R + k) * 3 + q];
-  A P[R][3];
-#pragma unroll
-  for (int r = 0; r < R; ++r) P[r][0] = P[r][1] = P[r][2] = 0;
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int L = (1 - c) * 4 + ((1 - b) ? ((1 - a) ? 2 : 3) : ((1 - a) ? 1 : 0));
-    const T* K0 = Ke + (3 * L + 0) * 24;
-    const T* K1 = Ke + (3 * L + 1) * 24;
-    const T* K2 = Ke + (3 * L + 2) * 24;
-    T t[R][3];
-#pragma unroll
-    for (int r = 0; r < R; ++r) t[r][0] = t[r][1] = t[r][2] = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int px = lpos(j, 0), py = lpos(j, 1), pz = lpos(j, 2);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const T k0 = K0[3 * j + q], k1 = K1[3 * j + q], k2 = K2[3 * j + q];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const T uu = w[px][py][r + c + pz][q];
-          t[r][0] += k0 * uu;
-          t[r][1] += k1 * uu;
-          t[r][2] += k2 * uu;
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const T e = es[(a * EY + (ly + b)) * EZ + (lz * R + r + c)];
-      P[r][0] += (A)(e * t[r][0]);
-      P[r][1] += (A)(e * t[r][1]);
-      P[r][2] += (A)(e * t[r][2]);
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) part[wv][r * 3 + q][lane] = P[r][q];
-  __syncthreads();
-  // outputs in DOF order: o = ly * NOUT + (zz * 3 + q), zz = lz * R + r
-  for (int o = threadIdx.x; o < TY * NOUT; o += kBlock) {
-    const int oly = o / NOUT, zi = o - oly * NOUT;
-    const int zz = zi / 3, q = zi - zz * 3;
-    const int iy = y0 + oly, iz = z0 + zz;
-    if ((iy >= g.NY) | (iz >= g.NZ)) continue;
-    const int f = map_idx<OFF>(map, 3 * ((ix * g.NY + iy) * g.NZ + iz) + q);
-    if (f < 0) continue;
-    const int ln = oly * TZ + zz / R, s = (zz % R) * 3 + q;
-    const A v = (part[0][s][ln] + part[1][s][ln]) + (part[2][s][ln] + part[3][s][ln]);
-    apply_epilogue<T, EPI>(f, (T)v, u, y, ep);
-  }
-}
-
 // Element strain energy and SIMP sensitivity (solver_v4.py:258-265):
 //   ce_e = u_e^T KE u_e  (unit modulus; KUe_j = sum_i u_i KE_ij in i order,
 //          then ce = sum_j KUe_j u_j in j order),
@@ -512,9 +370,7 @@ int elem_sensitivity(const Grid& g, const DofMap& map, const double* Ke, const d
 
 static std::atomic<int> g_split_f64{4}, g_split_f32{4};
 
-static bool valid_split(int s) {
-  return s == 1 || s == 4 || s == 8 || s == 16 || s == 82 || s == 83 || s == 42;
-}
+static bool valid_split(int s) { return s == 1 || s == 4 || s == 8 || s == 16; }
 
 void set_fine_split(int split_f64, int split_f32) {
   if (valid_split(split_f64)) g_split_f64 = split_f64;
@@ -529,19 +385,6 @@ static void launch_fine_m(int split, const Grid& g, const DofMap& map, const T* 
   if (split == 1) {
     hipLaunchKernelGGL((k_fine_matvec_node<T, EPI, OFF>), dim3(cdiv(g.n_nodes, kBlock)),
                        dim3(kBlock), 0, st, g, map, Ke, E, u, y, ep, gate);
-  } else if (split == 82 || split == 83 || split == 42) {
-    // register-blocked LDS tiles: code 10*TZ + R
-    const int tz = split / 10, r = split % 10, ty = 64 / tz;
-    const int tiles = g.NX * cdiv(g.NY, ty) * cdiv(g.NZ, tz * r);
-    if (split == 82)
-      hipLaunchKernelGGL((k_fine_matvec_zblk<T, EPI, OFF, 8, 2>), dim3(tiles), dim3(kBlock), 0,
-                         st, g, map, Ke, E, u, y, ep, gate);
-    else if (split == 83)
-      hipLaunchKernelGGL((k_fine_matvec_zblk<T, EPI, OFF, 8, 3>), dim3(tiles), dim3(kBlock), 0,
-                         st, g, map, Ke, E, u, y, ep, gate);
-    else
-      hipLaunchKernelGGL((k_fine_matvec_zblk<T, EPI, OFF, 4, 2>), dim3(tiles), dim3(kBlock), 0,
-                         st, g, map, Ke, E, u, y, ep, gate);
   } else if (split == 8 || split == 16) {
     const int tz = split, ty = 64 / tz;
     const int tiles = g.NX * cdiv(g.NY, ty) * cdiv(g.NZ, tz);

@@ -24,9 +24,8 @@ struct Work {
 constexpr int kSweepMaxBlocks = 1024;
 
 // Fine-operator launch shapes timed by the autotuner, in this order: 1 lane
-// per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16, register-blocked
-// LDS tiles 82 (8 lanes x 2 nodes along z), 83 (8 x 3) and 42 (4 x 2).
-constexpr int kFineShapes = 7;
+// per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16.
+constexpr int kFineShapes = 4;
 
 // Epilogue applied by the operator kernels to each output DOF i:
 //   PLAIN : y_i = (A x)_i

@@ -371,7 +371,7 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
   hipEvent_t e0, e1;
   GF_HIP(hipEventCreate(&e0));
   GF_HIP(hipEventCreate(&e1));
-  const int shapes[kFineShapes] = {1, 4, 8, 16, 82, 83, 42};
+  const int shapes[kFineShapes] = {1, 4, 8, 16};
   float best = 1e30f;
   int best_shape = 4;
   for (int si = 0; si < kFineShapes; ++si) {

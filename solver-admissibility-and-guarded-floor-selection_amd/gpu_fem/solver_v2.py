@@ -83,8 +83,7 @@ class FineOperator:
 
 
 # fine-operator launch shapes in the autotuner's timing order: (name, code)
-LAUNCH_SHAPES = (("node", 1), ("wave4", 4), ("tile8x8", 8), ("tile4x16", 16),
-                 ("zblk8x2", 82), ("zblk8x3", 83), ("zblk4x2", 42))
+LAUNCH_SHAPES = (("node", 1), ("wave4", 4), ("tile8x8", 8), ("tile4x16", 16))
 
 
 class MatrixFreeKff:
@@ -126,8 +125,7 @@ class MatrixFreeKff:
         return self.launch_shape
 
     def set_launch_shape(self, shape: int):
-        """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles,
-        82/83/42 register-blocked LDS tiles)."""
+        """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles)."""
         _lib.call("gf_ctx_set_shape", self._ctx.h, int(shape))
         self.launch_shape = int(shape)
 

@@ -307,8 +307,7 @@ def test_coarsest_inverse_matches_triangular(dims, rho_min):
                                        ((12, 30, 6), "tip")])
 def test_fine_launch_shapes_bitwise_identical(dims, kind):
     """Every fine-operator launch shape (lane per node, four waves per 64
-    nodes, LDS tiles 8x8 and 4x16, register-blocked LDS tiles with 2 or 3
-    z-nodes per lane) gives the same bits, FP64 and FP32, with
+    nodes, LDS tiles 8x8 and 4x16) gives the same bits, FP64 and FP32, with
     the fused residual epilogue too (table- and offset-mode DOF maps, partial
     tiles)."""
     from gpu_fem import _lib
