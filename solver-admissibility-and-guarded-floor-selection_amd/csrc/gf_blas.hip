@@ -89,12 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_mgs_axpy_dot(long n, const double* _
 constexpr int kSweepBlock = 256;
 constexpr long kSweepSpinCap = 1L << 20;
 
-// PF: prefetch V_{s+1} into registers during the exchange of step s (PER
-// values per lane held three times); without it V_s is loaded at the start of
-// step s, which leaves room for twice as many values per lane.
-template <int PER, bool PF>
-__global__ __launch_bounds__(kSweepBlock)
-__attribute__((amdgpu_waves_per_eu(PF ? 1 : 3))) void k_mgs_sweep(long n, int j, int m,
+template <int PER>
+__global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
                                                           const double* __restrict__ V,
                                                           double* __restrict__ w, double* H,
                                                           double* part, int* flags, double* err,
@@ -113,7 +109,7 @@ __attribute__((amdgpu_waves_per_eu(PF ? 1 : 3))) void k_mgs_sweep(long n, int j,
   const unsigned f0 = f0s;
   const long stride = (long)G * kSweepBlock;
   const long base = (long)b * kSweepBlock + threadIdx.x;
-  double wr[PER], vr[PER], vn[PF ? PER : 1];
+  double wr[PER], vr[PER], vn[PER];
   bool broken = false;  // wave 0 only
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -123,14 +119,6 @@ __attribute__((amdgpu_waves_per_eu(PF ? 1 : 3))) void k_mgs_sweep(long n, int j,
   }
   for (int s = 0; s <= j + 1; ++s) {
     double a = 0.0;
-    if (!PF && s > 0 && s <= j) {
-      const double* vs = V + (long)s * n;
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const long i = base + k * stride;
-        vr[k] = i < n ? vs[i] : 0.0;
-      }
-    }
     if (s <= j) {
 #pragma unroll
       for (int k = 0; k < PER; ++k) a += wr[k] * vr[k];
@@ -139,12 +127,12 @@ __attribute__((amdgpu_waves_per_eu(PF ? 1 : 3))) void k_mgs_sweep(long n, int j,
       for (int k = 0; k < PER; ++k) a += wr[k] * wr[k];
     }
     const double bs = block_sum(a, lds);
-    if (PF && s < j) {  // prefetch V_{s+1} while the exchange is in flight
+    if (s < j) {  // prefetch V_{s+1} while the exchange is in flight
       const double* vs = V + (long)(s + 1) * n;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const long i = base + k * stride;
-        vn[(PF ? k : 0)] = i < n ? vs[i] : 0.0;
+        vn[k] = i < n ? vs[i] : 0.0;
       }
     }
     const int slot = (s & 1) * kSweepMaxBlocks;
@@ -183,9 +171,9 @@ __attribute__((amdgpu_waves_per_eu(PF ? 1 : 3))) void k_mgs_sweep(long n, int j,
 #pragma unroll
       for (int k = 0; k < PER; ++k) wr[k] -= h * vr[k];
       if (b == 0 && threadIdx.x == 0) H[(long)s * m + j] = h;
-      if (PF && s < j) {
+      if (s < j) {
 #pragma unroll
-        for (int k = 0; k < PER; ++k) vr[k] = vn[(PF ? k : 0)];
+        for (int k = 0; k < PER; ++k) vr[k] = vn[k];
       }
     } else {
       const double nrm = sqrt(h);
@@ -205,26 +193,32 @@ __global__ void k_fill_int(int n, int* p, int v) {
 }
 
 namespace {
-template <int PER, bool PF>
+// A sweep's blocks spin until every block of its grid has arrived, so the
+// whole grid must be resident at once.  Solver instances on other streams
+// may be running sweeps at the same moment (up to one per hardware queue,
+// four by default), so a grid may use at most a quarter of the device's
+// co-resident capacity; larger orthogonalisations use one launch per
+// projection.
+template <int PER>
 int sweep_fits(int G) {
   static std::atomic<int> max_blocks{-1};  // co-resident blocks of this instantiation
   if (max_blocks.load() < 0) {
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mgs_sweep<PER, PF>, kSweepBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mgs_sweep<PER>, kSweepBlock, 0) !=
         hipSuccess)
       return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       return 0;
     max_blocks.store(per_cu * cus);
   }
-  return G <= max_blocks.load();
+  return 4 * G <= max_blocks.load();
 }
 
-template <int PER, bool PF = true>
+template <int PER>
 int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H,
                  double* err, const int* gate, int cooperative, int G, hipStream_t st) {
-  if (!sweep_fits<PER, PF>(G)) return 0;
+  if (!sweep_fits<PER>(G)) return 0;
   if (wk.sweep_G != G) {
     // the sweep needs all flags of its grid equal at entry; flags left by a
     // different grid (or an unknown history) are reset first -- a plain
@@ -235,13 +229,13 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
   }
   if (cooperative) {
     void* args[] = {&n, &j, &m, (void*)&V, &wv, &H, &wk.partials, &wk.flags, &err, (void*)&gate};
-    GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER, PF>), dim3(G),
+    GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER>), dim3(G),
                                       dim3(kSweepBlock), args, 0, st));
   } else {
     // G is far below the device's co-resident capacity (checked above), so
     // every block is resident once the kernels ahead of it drain; the waits
     // are bounded regardless
-    hipLaunchKernelGGL((k_mgs_sweep<PER, PF>), dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
+    hipLaunchKernelGGL((k_mgs_sweep<PER>), dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
                        wk.partials, wk.flags, err, gate);
     GF_CHECK_LAUNCH();
   }
@@ -260,11 +254,7 @@ int mgs_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, doubl
   if (per <= 2) return launch_sweep<2>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
   if (per <= 4) return launch_sweep<4>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
   if (per <= 8) return launch_sweep<8>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
-  const int rc = launch_sweep<16>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
-  if (rc != 0) return rc;
-  // more blocks than can be co-resident with the prefetch buffer (n of a few
-  // million): the form without it needs fewer registers, so more blocks fit
-  return launch_sweep<16, false>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  return launch_sweep<16>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
 }
 
 // y = b - s*x ; out = ||y||  with s = *sptr (trial residual norm, y may be null)

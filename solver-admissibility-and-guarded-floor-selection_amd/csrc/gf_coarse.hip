@@ -1199,11 +1199,28 @@ void pcg_buffers_release(void* p, size_t bytes) {
   uc_free_list().push_back(UcBlock{static_cast<char*>(p), bytes});
 }
 
+// Co-resident blocks of k_pcg_persistent on this device (the kernel's blocks
+// wait for each other, so its grid must fit at once, with room for the grids
+// of other streams: at most a quarter of the capacity, as for the sweep).
+static int pcg_capacity() {
+  static std::atomic<int> cap{-1};
+  if (cap.load() < 0) {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<true>, kBlock, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    cap.store(per_cu * cus);
+  }
+  return cap.load();
+}
+
 int fixed_pcg_persistent(const Grid& g, const DofMap& map, const double* A, const double* dinv,
                          int n_free, int iters, const double* b, double* x, void* ucbuf,
                          double* err, const int* gate, hipStream_t st) {
   const int G = cdiv(g.n_nodes, 64);
-  if (G > kPcgMaxBlocks) return 0;
+  if (G > kPcgMaxBlocks || 4 * G > pcg_capacity()) return 0;
   int* flags = static_cast<int*>(ucbuf);
   double* part = reinterpret_cast<double*>(static_cast<char*>(ucbuf) + sizeof(int) * kPcgMaxBlocks);
   double* r = part + 2 * kPcgMaxBlocks;
