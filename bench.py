@@ -104,7 +104,9 @@ def parse():
     ap.add_argument("--theta-out", default="", help="save the trained parameters (.npy)")
     ap.add_argument("--batch-states", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
-                    help="time only the FP64 fine matvec loop (for rocprofv3 runs)")
+                    help="time only the fine matvec loop (for rocprofv3 runs)")
+    ap.add_argument("--roofline-kernel", default="fp64", choices=["fp64", "bf16"],
+                    help="with --roofline-only: the FP64 operator or the BF16 MFMA form")
     return ap.parse_args()
 
 
@@ -566,6 +568,13 @@ def main():
     session = gpol.PolicySession(spec, workers=args.workers, **kw)
     runner = session.runners[0]
     gen = states(args)
+    if args.roofline_only and args.roofline_kernel == "bf16":
+        # the BF16 matrix-core form of the fused operator (element contraction
+        # on MFMA + node gather), for its MFMA-utilisation counters
+        r = bf16_matvec_rate(runner, n_launch=2000)
+        r.update(grid=list(dims), n_elem=runner.mf_op._n_elem, flops_per_launch=1152 * runner.mf_op._n_elem)
+        print(json.dumps(r))
+        return
     if args.roofline_only:
         avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner, n_launch=2000)
         # calibration for HBM counters: a plain device copy of known size
