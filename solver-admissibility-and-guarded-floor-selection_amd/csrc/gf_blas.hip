@@ -36,11 +36,14 @@ __global__ __launch_bounds__(kBlock) void k_dot(long n, const T* __restrict__ x,
   finish_reduce<1>(a, partials, ticket, out, post);
 }
 
-// out[0]=x.x  out[1]=y.y  out[2]=z.z  out[3]=y.z  (line-search statistics)
+// out[0]=x.x  out[1]=y.y  out[2]=z.z  out[3]=y.z  (line-search statistics);
+// with `step` set the folding block also forms the line-search step
+// (ls_scale) into out[4], out[5]
 __global__ __launch_bounds__(kBlock) void k_dot4_ls(long n, const double* __restrict__ c,
                                                     const double* __restrict__ r,
                                                     const double* __restrict__ Ac,
                                                     double* partials, int* ticket, double* out,
+                                                    int step, double max_scale,
                                                     const int* gate) {
   if (gate && *gate == 0) return;
   double a[4] = {0.0, 0.0, 0.0, 0.0};
@@ -52,7 +55,8 @@ __global__ __launch_bounds__(kBlock) void k_dot4_ls(long n, const double* __rest
     a[2] += av * av;
     a[3] += rv * av;
   }
-  finish_reduce<4>(a, partials, ticket, out, 0);
+  if (finish_reduce<4>(a, partials, ticket, out, 0) && step && threadIdx.x == 0)
+    ls_scale(out, max_scale);
 }
 
 // w -= h*v (h = *hptr) and out = next . w  (fused MGS step)
@@ -94,7 +98,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
                                                           const double* __restrict__ V,
                                                           double* __restrict__ w, double* H,
                                                           double* part, int* flags, double* err,
-                                                          const int* gate) {
+                                                          const int* gate, GivensArgs gv) {
   if (gate && *gate == 0) return;  // uniform: written by an earlier kernel
   __shared__ double lds[4];
   __shared__ double hsh;
@@ -177,7 +181,11 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
       }
     } else {
       const double nrm = sqrt(h);
-      if (b == 0 && threadIdx.x == 0) H[(long)(j + 1) * m + j] = nrm;
+      if (b == 0 && threadIdx.x == 0) {
+        H[(long)(j + 1) * m + j] = nrm;
+        // the Givens update of this step, in place of a separate launch
+        if (gv.fl) givens_step(m, j, gv, H);
+      }
       const bool scale = nrm > 1e-14;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
@@ -217,7 +225,9 @@ int sweep_fits(int G) {
 
 template <int PER>
 int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H,
-                 double* err, const int* gate, int cooperative, int G, hipStream_t st) {
+                 double* err, const int* gate, int cooperative, const GivensArgs* gvp, int G,
+                 hipStream_t st) {
+  GivensArgs gv = gvp ? *gvp : GivensArgs{0.0, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (!sweep_fits<PER>(G)) return 0;
   if (wk.sweep_G != G) {
     // the sweep needs all flags of its grid equal at entry; flags left by a
@@ -228,7 +238,8 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
     wk.sweep_G = G;
   }
   if (cooperative) {
-    void* args[] = {&n, &j, &m, (void*)&V, &wv, &H, &wk.partials, &wk.flags, &err, (void*)&gate};
+    void* args[] = {&n,   &j,   &m,           (void*)&V,  &wv, &H, &wk.partials, &wk.flags, &err,
+                    (void*)&gate, &gv};
     GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER>), dim3(G),
                                       dim3(kSweepBlock), args, 0, st));
   } else {
@@ -236,7 +247,7 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
     // every block is resident once the kernels ahead of it drain; the waits
     // are bounded regardless
     hipLaunchKernelGGL((k_mgs_sweep<PER>), dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
-                       wk.partials, wk.flags, err, gate);
+                       wk.partials, wk.flags, err, gate, gv);
     GF_CHECK_LAUNCH();
   }
   return 1;
@@ -244,17 +255,17 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
 }  // namespace
 
 int mgs_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H, double* err,
-              const int* gate, int cooperative, hipStream_t st) {
+              const int* gate, int cooperative, const GivensArgs* gv, hipStream_t st) {
   constexpr int kPerMax = 16;
   int G = cdiv(n, (long)kSweepBlock * kPerMax);
   if (G < 1) G = 1;
   if (G > kSweepMaxBlocks) return 0;
   const long per = (n + (long)G * kSweepBlock - 1) / ((long)G * kSweepBlock);
-  if (per <= 1) return launch_sweep<1>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
-  if (per <= 2) return launch_sweep<2>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
-  if (per <= 4) return launch_sweep<4>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
-  if (per <= 8) return launch_sweep<8>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
-  return launch_sweep<16>(wk, n, V, j, m, wv, H, err, gate, cooperative, G, st);
+  if (per <= 1) return launch_sweep<1>(wk, n, V, j, m, wv, H, err, gate, cooperative, gv, G, st);
+  if (per <= 2) return launch_sweep<2>(wk, n, V, j, m, wv, H, err, gate, cooperative, gv, G, st);
+  if (per <= 4) return launch_sweep<4>(wk, n, V, j, m, wv, H, err, gate, cooperative, gv, G, st);
+  if (per <= 8) return launch_sweep<8>(wk, n, V, j, m, wv, H, err, gate, cooperative, gv, G, st);
+  return launch_sweep<16>(wk, n, V, j, m, wv, H, err, gate, cooperative, gv, G, st);
 }
 
 // y = b - s*x ; out = ||y||  with s = *sptr (trial residual norm, y may be null)
@@ -409,7 +420,15 @@ int dot_f32(Work& w, long n, const float* x, const float* y, double* out, bool s
 int dot4_ls(Work& w, long n, const double* c, const double* r, const double* Ac, double* out4,
             const int* gate, hipStream_t st) {
   hipLaunchKernelGGL(k_dot4_ls, dim3(red_grid(n)), dim3(kBlock), 0, st, n, c, r, Ac, w.partials,
-                     w.ticket, out4, gate);
+                     w.ticket, out4, 0, 0.0, gate);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
+int dot4_ls_step(Work& w, long n, const double* c, const double* r, const double* Ac,
+                 double* out6, double max_scale, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot4_ls, dim3(red_grid(n)), dim3(kBlock), 0, st, n, c, r, Ac, w.partials,
+                     w.ticket, out6, 1, max_scale, gate);
   GF_CHECK_LAUNCH();
   return 0;
 }

@@ -52,6 +52,11 @@ int dot_f32(Work& w, long n, const float* x, const float* y, double* out, bool s
             const int* gate, hipStream_t st);
 int dot4_ls(Work& w, long n, const double* c, const double* r, const double* Ac, double* out4,
             const int* gate, hipStream_t st);
+// dot4_ls and, in the same launch, the residual line-search step from those
+// statistics (ls_scale): out6[0..3] statistics, out6[4] alpha, out6[5]
+// rejection flag.
+int dot4_ls_step(Work& w, long n, const double* c, const double* r, const double* Ac,
+                 double* out6, double max_scale, const int* gate, hipStream_t st);
 int mgs_axpy_dot(Work& w, long n, const double* v, double* wv, const double* h,
                  const double* next, double* out, bool sqrt_, const int* gate, hipStream_t st);
 // One Arnoldi orthogonalisation by modified Gram-Schmidt in a single
@@ -62,8 +67,68 @@ int mgs_axpy_dot(Work& w, long n, const double* v, double* wv, const double* h,
 // record in a hipGraph (no launch-time state).  Returns 1 when the sweep was
 // launched, 0 when n is outside the register-resident range (the caller then
 // uses mgs_axpy_dot per projection), negative on error.
+// Device state of one FGMRES instance that the Givens update of Arnoldi step
+// j reads and writes (Krylov::cs, sn, g, sc, fl; see givens_step).
+struct GivensArgs {
+  double tol;
+  double* cs;
+  double* sn;
+  double* g;
+  double* sc;
+  int* fl;
+};
+
+// Givens update of Hessenberg column j (multigrid_v4.py:495-515): apply the
+// previous rotations, form the new one (cs = H_jj/rho, sn = H_{j+1,j}/rho),
+// rotate g, record the projected residual |g_{j+1}|/||b|| and h_{j+1,j} in
+// sc[2], sc[3], count the step in fl[4], fl[2] = j+1, and on convergence or
+// breakdown (projected residual <= tol or h_{j+1,j} <= 1e-14) set fl[3] and
+// clear the step gate fl[0].  One thread; skipped while fl[0] == 0.
+__device__ __forceinline__ void givens_step(int m, int j, const GivensArgs& a, double* H) {
+  int* fl = a.fl;
+  if (!fl[0]) return;
+  double* cs = a.cs;
+  double* sn = a.sn;
+  double* g = a.g;
+  const double hn = H[(j + 1) * m + j];
+  for (int i = 0; i < j; ++i) {
+    const double hij = H[i * m + j], hi1 = H[(i + 1) * m + j];
+    const double tmp = cs[i] * hij + sn[i] * hi1;
+    H[(i + 1) * m + j] = -sn[i] * hij + cs[i] * hi1;
+    H[i * m + j] = tmp;
+  }
+  const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
+  const double denom = sqrt(hjj * hjj + hj1 * hj1);
+  double c, s;
+  if (denom < 1e-300) {
+    c = 1.0;
+    s = 0.0;
+  } else {
+    c = hjj / denom;
+    s = hj1 / denom;
+  }
+  cs[j] = c;
+  sn[j] = s;
+  H[j * m + j] = c * hjj + s * hj1;
+  H[(j + 1) * m + j] = 0.0;
+  const double gj = g[j];
+  g[j + 1] = -s * gj;
+  g[j] = c * gj;
+  fl[4] += 1;
+  fl[2] = j + 1;
+  const double res = fabs(g[j + 1]) / a.sc[0];
+  a.sc[2] = res;
+  a.sc[3] = hn;
+  if (res <= a.tol || hn <= 1e-14) {
+    fl[3] = 1;
+    fl[0] = 0;
+  }
+}
+
+// When gv is non-null the sweep also performs the Givens update of step j in
+// its last block-0 thread (givens_step), in place of a separate launch.
 int mgs_sweep(Work& w, long n, const double* V, int j, int m, double* wv, double* H, double* err,
-              const int* gate, int cooperative, hipStream_t st);
+              const int* gate, int cooperative, const GivensArgs* gv, hipStream_t st);
 int trial_norm(Work& w, long n, const double* b, const double* x, const double* s, double* y,
                double* out, const int* gate, hipStream_t st);
 int axpby_dev(long n, const double* a, double asign, const double* x, const double* b, double* y,

@@ -32,7 +32,7 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 // Fold NP partial sums of every block into out[0..NP) (last block only).
 // Returns true in the block that performed the fold.
 template <int NP>
-__device__ __forceinline__ void finish_reduce(const double (&v)[NP], double* partials,
+__device__ __forceinline__ bool finish_reduce(const double (&v)[NP], double* partials,
                                               int* ticket, double* out, int post) {
   __shared__ double lds[4];
   __shared__ bool last;
@@ -54,7 +54,7 @@ __device__ __forceinline__ void finish_reduce(const double (&v)[NP], double* par
     last = (t == (int)gridDim.x - 1);
   }
   __syncthreads();
-  if (!last) return;
+  if (!last) return false;
   const int nb = gridDim.x;
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -69,6 +69,23 @@ __device__ __forceinline__ void finish_reduce(const double (&v)[NP], double* par
     }
   }
   if (threadIdx.x == 0) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Residual line-search step (multigrid_v4.py:867-931) from the statistics
+// st[0] = c.c, st[1] = r.r, st[2] = Ac.Ac, st[3] = r.Ac: st[4] = alpha =
+// clip((r.Ac)/(Ac.Ac), 0, max_scale), st[5] = 1 when the correction is
+// rejected outright (a zero norm), else 0.  One thread.
+__device__ __forceinline__ void ls_scale(double* st, double max_scale) {
+  const double cn = sqrt(st[0]), rn = sqrt(st[1]), den = st[2];
+  if (cn <= 1e-300 || rn <= 1e-300 || den <= 1e-300) {
+    st[4] = 0.0;
+    st[5] = 1.0;
+    return;
+  }
+  const double raw = st[3] / den;
+  st[4] = fmin(max_scale, fmax(0.0, raw));
+  st[5] = 0.0;
 }
 
 // Reduction grid: ~4096 elements per block, between 16 and kRedBlocks blocks

@@ -129,40 +129,7 @@ __global__ void k_copy_scalar(const double* a, double* b) { *b = *a; }
 // sc[3] <- h_{j+1,j} before rotation.
 __global__ void k_givens(int m, int j, double tol, double* H, double* cs, double* sn, double* g,
                          double* sc, int* fl) {
-  if (!fl[0]) return;
-  const double hn = H[(j + 1) * m + j];
-  for (int i = 0; i < j; ++i) {
-    const double hij = H[i * m + j], hi1 = H[(i + 1) * m + j];
-    const double tmp = cs[i] * hij + sn[i] * hi1;
-    H[(i + 1) * m + j] = -sn[i] * hij + cs[i] * hi1;
-    H[i * m + j] = tmp;
-  }
-  const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
-  const double denom = sqrt(hjj * hjj + hj1 * hj1);
-  double c, s;
-  if (denom < 1e-300) {
-    c = 1.0;
-    s = 0.0;
-  } else {
-    c = hjj / denom;
-    s = hj1 / denom;
-  }
-  cs[j] = c;
-  sn[j] = s;
-  H[j * m + j] = c * hjj + s * hj1;
-  H[(j + 1) * m + j] = 0.0;
-  const double gj = g[j];
-  g[j + 1] = -s * gj;
-  g[j] = c * gj;
-  fl[4] += 1;
-  fl[2] = j + 1;
-  const double res = fabs(g[j + 1]) / sc[0];
-  sc[2] = res;
-  sc[3] = hn;
-  if (res <= tol || hn <= 1e-14) {
-    fl[3] = 1;
-    fl[0] = 0;
-  }
+  givens_step(m, j, GivensArgs{tol, cs, sn, g, sc, fl}, H);
 }
 
 // y = R^{-1} g over the leading jfinal x jfinal triangle (column sweep,
@@ -189,19 +156,6 @@ __global__ __launch_bounds__(256) void k_backsolve(int m, const double* H, const
 
 // ---- safeguarded corrections -----------------------------------------------
 // st: [0]=c.c [1]=r.r [2]=Ac.Ac [3]=r.Ac [4]=scale [5]=degenerate [6]=trial norm
-__global__ void k_ls_scalar(double* st, double max_scale, const int* gate) {
-  if (gate && *gate == 0) return;
-  const double cn = sqrt(st[0]), rn = sqrt(st[1]), den = st[2];
-  if (cn <= 1e-300 || rn <= 1e-300 || den <= 1e-300) {
-    st[4] = 0.0;
-    st[5] = 1.0;
-    return;
-  }
-  const double raw = st[3] / den;
-  st[4] = fmin(max_scale, fmax(0.0, raw));
-  st[5] = 0.0;
-}
-
 __global__ void k_ls_apply(long n, double* __restrict__ x, const double* __restrict__ c,
                            const double* st, double ratio, const double* __restrict__ r,
                            double* __restrict__ r2, const int* gate) {
@@ -917,8 +871,8 @@ int Gmg::correct(int l, double* x, const double* residual, const double* corr,
     return 0;
   }
   GF_TRY(level_op(l, corr, v.AC, EPI_PLAIN, EpiArgs<double>{}, gate, st));
-  GF_TRY(dot4_ls(work, n, corr, residual, v.AC, v.st, gate, st));
-  LAUNCH1(k_ls_scalar, st, v.st, p.coarse_correction_max_scale, gate);
+  GF_TRY(dot4_ls_step(work, n, corr, residual, v.AC, v.st, p.coarse_correction_max_scale, gate,
+                      st));
   GF_TRY(trial_norm(work, n, residual, v.AC, v.st + 4, residual_out, v.st + 6, gate, st));
   LAUNCH(k_ls_apply, n, st, n, x, corr, v.st, p.coarse_correction_acceptance_ratio, residual,
          residual_out, gate);
@@ -1038,7 +992,8 @@ int Gmg::fgmres_dev(Krylov& K, const std::function<int(const double*, double*, c
       GF_TRY(level_op(0, Zj, w, EPI_PLAIN, EpiArgs<double>{}, step, st));
       int swept = 0;
       if (p.mgs_sweep) {  // plain launch: this sequence is recorded in a graph
-        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, step, 0, st);
+        const GivensArgs gv{tol, K.cs, K.sn, K.g, K.sc, K.fl};
+        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, step, 0, &gv, st);
         if (swept < 0) return swept;
       }
       if (!swept) {
@@ -1051,7 +1006,7 @@ int Gmg::fgmres_dev(Krylov& K, const std::function<int(const double*, double*, c
                             K.H + (long)(j + 1) * m + j, true, step, st));
         GF_TRY(scale_inv(n, w, K.H + (long)(j + 1) * m + j, 1e-14, w, step, st));
       }
-      LAUNCH1(k_givens, st, m, j, tol, K.H, K.cs, K.sn, K.g, K.sc, K.fl);
+      if (!swept) LAUNCH1(k_givens, st, m, j, tol, K.H, K.cs, K.sn, K.g, K.sc, K.fl);
     }
     hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(256), 0, st, m, K.H, K.g, K.y, K.fl);
     GF_CHECK_LAUNCH();
@@ -1415,7 +1370,8 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
       GF_TRY(level_op(0, Zj, w, EPI_PLAIN, EpiArgs<double>{}, nullptr, st));
       int swept = 0;
       if (p.mgs_sweep) {
-        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, nullptr, p.mgs_sweep == 2, st);
+        const GivensArgs gv{tol, K.cs, K.sn, K.g, K.sc, K.fl};
+        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, nullptr, p.mgs_sweep == 2, &gv, st);
         if (swept < 0) return swept;
       }
       if (!swept) {
@@ -1428,7 +1384,7 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
                             K.H + (long)(j + 1) * m + j, true, nullptr, st));
         GF_TRY(scale_inv(n, w, K.H + (long)(j + 1) * m + j, 1e-14, w, nullptr, st));
       }
-      LAUNCH1(k_givens, st, m, j, tol, K.H, K.cs, K.sn, K.g, K.sc, K.fl);
+      if (!swept) LAUNCH1(k_givens, st, m, j, tol, K.H, K.cs, K.sn, K.g, K.sc, K.fl);
       // projected residual, h_{j+1,j} and the sweep error flag in one wait
       GF_HIP(hipMemcpyAsync(hs, K.sc + 2, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
       GF_HIP(hipMemcpyAsync(hs + 2, d_err, sizeof(double), hipMemcpyDeviceToHost, st));

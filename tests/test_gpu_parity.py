@@ -458,7 +458,11 @@ def test_concurrent_session_equals_sequential():
 def test_mgs_sweep_matches_per_projection_launches(dims, rho_min):
     """The outer FGMRES orthogonalisation in one cooperative launch per
     Arnoldi step gives the same solve as one launch per projection: equal
-    iteration counts and flags, histories equal up to the partial-sum grouping."""
+    iteration counts and flags, histories equal up to the partial-sum grouping
+    -- to 1e-10 at steps 0-1; later steps drift as the nonlinear
+    preconditioner amplifies the grouping (measured 1.5e-10, 4e-10, 2e-9 at
+    steps 2-4; the oracle against itself with a reordered matvec drifts by
+    2e-9 at step 2), so the whole history is held to 20 %."""
     from gpu_fem import policy as gpol
     from gpu_fem.presets import cantilever
 
@@ -473,7 +477,7 @@ def test_mgs_sweep_matches_per_projection_launches(dims, rho_min):
     (h1, i1, c1, x1), (h2, i2, c2, x2) = out
     assert i1 == i2 and c1 == c2
     rel = np.abs(h1 - h2) / np.abs(h2)
-    assert np.all(rel[:3] <= 1e-10), rel[:5]
+    assert np.all(rel[:2] <= 1e-10), rel[:5]
     assert np.all(rel <= 0.2)
 
 
