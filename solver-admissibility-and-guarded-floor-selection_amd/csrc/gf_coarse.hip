@@ -592,58 +592,179 @@ int stencil_to_dense(const Grid& g, const DofMap& map, const double* A, int n_fr
   return 0;
 }
 
-// Right-looking Cholesky in one work-group (n <= 5000): for each pivot k the
-// column below it is scaled, then the trailing lower triangle is updated with
-// one thread per matrix entry (consecutive threads on consecutive columns, so
-// the row-major updates coalesce).
-__global__ __launch_bounds__(1024) void k_cholesky(int n, double* __restrict__ D,
-                                                   double* __restrict__ Lt, int* fail) {
+// Blocked right-looking Cholesky (lower factor, row-major, n <= 5000) over
+// 64-wide panels, many work-groups per panel.  Every entry receives exactly
+// the updates of the unblocked right-looking algorithm, in the same order:
+// for pivot k ascending, D[i][j] -= D[i][k] * D[j][k] (k < j <= i), then the
+// column below a pivot is divided by it.  Per panel [kb, kb+m):
+//   k_potrf_diag  factors the m x m diagonal block in LDS (one work-group);
+//   k_trsm_rows   solves the panel rows below it, one row per thread, by
+//                 forward substitution against the diagonal block (the
+//                 updates from this panel's pivots, ascending, then the
+//                 division);
+//   k_syrk_tiles  applies this panel's pivots, ascending, to the trailing
+//                 lower triangle in 64x64 tiles.
+// A non-positive or non-finite pivot sets *fail and every later kernel of
+// the factorisation returns at once.
+constexpr int kCB = 64;
+
+__global__ __launch_bounds__(256) void k_potrf_diag(int n, double* __restrict__ D, int kb,
+                                                    int* fail) {
+  __shared__ double T[kCB][kCB + 1];
   __shared__ double piv;
   __shared__ int bad;
-  if (threadIdx.x == 0) bad = 0;
+  if (*fail) return;
+  const int m = min(kCB, n - kb);
+  const int tid = threadIdx.x;
+  for (int e = tid; e < m * m; e += blockDim.x) {
+    const int r = e / m, c = e % m;
+    T[r][c] = c <= r ? D[(long)(kb + r) * n + kb + c] : 0.0;
+  }
+  if (tid == 0) bad = 0;
   __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    if (threadIdx.x == 0) {
-      const double d = D[(long)k * n + k];
+  for (int k = 0; k < m; ++k) {
+    if (tid == 0) {
+      const double d = T[k][k];
       if (!(d > 0.0) || !isfinite(d)) bad = 1;
       piv = sqrt(d);
-      D[(long)k * n + k] = piv;
+      T[k][k] = piv;
     }
     __syncthreads();
     if (bad) break;
     const double p = piv;
-    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) D[(long)i * n + k] /= p;
+    for (int i = k + 1 + tid; i < m; i += blockDim.x) T[i][k] /= p;
     __syncthreads();
-    const int m = n - k - 1;  // trailing size
-    // entries (i, j), k < j <= i < n, enumerated row by row
-    const long total = (long)m * (m + 1) / 2;
-    for (long e = threadIdx.x; e < total; e += blockDim.x) {
-      // row r (0-based in the trailing block) with r(r+1)/2 <= e < (r+1)(r+2)/2
-      long r = (long)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
-      while ((r + 1) * (r + 2) / 2 <= e) ++r;
-      while (r * (r + 1) / 2 > e) --r;
-      const long c = e - r * (r + 1) / 2;
-      const int i = k + 1 + (int)r, j = k + 1 + (int)c;
-      D[(long)i * n + j] -= D[(long)i * n + k] * D[(long)j * n + k];
+    const int mm = m - k - 1;
+    for (int e = tid; e < mm * mm; e += blockDim.x) {
+      const int i = k + 1 + e / mm, j = k + 1 + e % mm;
+      if (j <= i) T[i][j] -= T[i][k] * T[j][k];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *fail = bad;
-  __syncthreads();
-  if (bad) return;
-  for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
-    const int i = (int)(idx / n), j = (int)(idx % n);
-    if (j > i) D[idx] = 0.0;
+  if (bad) {
+    if (tid == 0) *fail = 1;
+    return;
   }
-  __syncthreads();
-  for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
-    const int i = (int)(idx / n), j = (int)(idx % n);
-    Lt[(long)j * n + i] = D[idx];
+  for (int e = tid; e < m * m; e += blockDim.x) {
+    const int r = e / m, c = e % m;
+    if (c <= r) D[(long)(kb + r) * n + kb + c] = T[r][c];
   }
 }
 
+__global__ __launch_bounds__(kCB) void k_trsm_rows(int n, double* __restrict__ D, int kb,
+                                                   const int* fail) {
+  __shared__ double L11[kCB][kCB + 1];
+  if (*fail) return;
+  const int m = min(kCB, n - kb);
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int r = e / m, c = e % m;
+    L11[r][c] = c <= r ? D[(long)(kb + r) * n + kb + c] : 0.0;
+  }
+  __syncthreads();
+  const int i = kb + m + blockIdx.x * kCB + threadIdx.x;
+  if (i >= n) return;
+  double* row = D + (long)i * n + kb;
+  double x[kCB];
+#pragma unroll
+  for (int c = 0; c < kCB; ++c) x[c] = c < m ? row[c] : 0.0;
+#pragma unroll
+  for (int c = 0; c < kCB; ++c) {
+    if (c < m) {
+#pragma unroll
+      for (int l = 0; l < c; ++l) x[c] -= x[l] * L11[c][l];
+      x[c] /= L11[c][c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kCB; ++c)
+    if (c < m) row[c] = x[c];
+}
+
+// Tiles (ti, tj), tj <= ti, of the trailing block [t0, n)^2, t0 = kb + m;
+// tile index q enumerates the lower triangle row by row.  Each thread owns a
+// 4x4 set of entries; the panel columns of the tile's rows and columns are
+// staged in LDS.
+__global__ __launch_bounds__(256) void k_syrk_tiles(int n, double* __restrict__ D, int kb,
+                                                    const int* fail) {
+  __shared__ double Ai[kCB][kCB + 1];
+  __shared__ double Aj[kCB][kCB + 1];
+  if (*fail) return;
+  const int m = min(kCB, n - kb);
+  const int t0 = kb + m;
+  const int q = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= q) ++ti;
+  while (ti * (ti + 1) / 2 > q) --ti;
+  const int tj = q - ti * (ti + 1) / 2;
+  const int r0 = t0 + ti * kCB, c0 = t0 + tj * kCB;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kCB * kCB; e += blockDim.x) {
+    const int r = e / kCB, l = e % kCB;
+    Ai[r][l] = (r0 + r < n && l < m) ? D[(long)(r0 + r) * n + kb + l] : 0.0;
+    Aj[r][l] = (c0 + r < n && l < m) ? D[(long)(c0 + r) * n + kb + l] : 0.0;
+  }
+  __syncthreads();
+  const int rr = (tid / 16) * 4, cc = (tid % 16) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = r0 + rr + a, j = c0 + cc + b;
+      acc[a][b] = (i < n && j < n && j <= i) ? D[(long)i * n + j] : 0.0;
+    }
+  for (int l = 0; l < m; ++l) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) av[a] = Ai[rr + a][l];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) bv[b] = Aj[cc + b][l];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] -= av[a] * bv[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = r0 + rr + a, j = c0 + cc + b;
+      if (i < n && j < n && j <= i) D[(long)i * n + j] = acc[a][b];
+    }
+}
+
+// upper triangle zeroed, Lt = L^T (row-major); skipped after a failure
+__global__ __launch_bounds__(kBlock) void k_chol_finish(int n, double* __restrict__ D,
+                                                        double* __restrict__ Lt,
+                                                        const int* fail) {
+  if (*fail) return;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const double v = j > i ? 0.0 : D[e];
+  D[e] = v;
+  Lt[(long)j * n + i] = v;
+}
+
+__global__ void k_clear_flag(int* fail) { *fail = 0; }
+
 int dense_cholesky(int n, double* D, double* Lt, int* fail, hipStream_t st) {
-  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1024), 0, st, n, D, Lt, fail);
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, st, fail);
+  GF_CHECK_LAUNCH();
+  for (int kb = 0; kb < n; kb += kCB) {
+    const int m = std::min(kCB, n - kb);
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, st, n, D, kb, fail);
+    GF_CHECK_LAUNCH();
+    const int rows = n - kb - m;
+    if (rows <= 0) break;
+    hipLaunchKernelGGL(k_trsm_rows, dim3(cdiv(rows, kCB)), dim3(kCB), 0, st, n, D, kb, fail);
+    GF_CHECK_LAUNCH();
+    const int T = cdiv(rows, kCB);
+    hipLaunchKernelGGL(k_syrk_tiles, dim3(T * (T + 1) / 2), dim3(256), 0, st, n, D, kb, fail);
+    GF_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_chol_finish, dim3(cdiv((long)n * n, kBlock)), dim3(kBlock), 0, st, n, D,
+                     Lt, fail);
   GF_CHECK_LAUNCH();
   return 0;
 }
@@ -758,53 +879,167 @@ __global__ void k_recip_diag(int n, const double* __restrict__ L, double* __rest
     r[i] = 1.0 / L[(long)i * n + i];
 }
 
-// Y = L^{-1}: column j of Y by forward substitution of e_j (one thread per
-// column; Y stored row-major, Y[i][j] for i >= j).
-__global__ __launch_bounds__(kBlock) void k_tri_inverse(int n, const double* __restrict__ L,
-                                                        double* __restrict__ Y) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  for (int i = 0; i < j; ++i) Y[(long)i * n + j] = 0.0;
-  for (int i = j; i < n; ++i) {
-    double acc = (i == j) ? 1.0 : 0.0;
-    const double* row = L + (long)i * n;
-    for (int k = j; k < i; ++k) acc -= row[k] * Y[(long)k * n + j];
-    Y[(long)i * n + j] = acc / row[i];
+// Y = L^{-1} (row-major, lower) by right-looking blocked forward substitution
+// of the identity over 64-row blocks.  Every entry receives the updates of
+// column-wise substitution, Y[i][j] = (delta_ij - sum_{k=j}^{i-1} L[i][k]
+// Y[k][j]) / L[i][i], in the same ascending k order (terms with k < j are
+// exact zeros), so the result does not depend on the blocking:
+//   k_inv_diag_rows  finishes the block's rows: thread j owns column j and
+//                    applies the block's own pivots (L block in LDS);
+//   k_inv_update     subtracts the finished block's contribution from every
+//                    later row, in 64x64 tiles.
+__global__ __launch_bounds__(kBlock) void k_eye(long nn, int n, double* __restrict__ Y) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  Y[e] = (e / n == e % n) ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(kCB) void k_inv_diag_rows(int n, const double* __restrict__ L,
+                                                       double* __restrict__ Y, int ib) {
+  __shared__ double Lb[kCB][kCB + 1];
+  const int m = min(kCB, n - ib);
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int r = e / m, c = e % m;
+    Lb[r][c] = c <= r ? L[(long)(ib + r) * n + ib + c] : 0.0;
   }
+  __syncthreads();
+  const int j = blockIdx.x * kCB + threadIdx.x;  // columns 0 .. ib+m-1
+  if (j >= ib + m) return;
+  double x[kCB];
+#pragma unroll
+  for (int r = 0; r < kCB; ++r) x[r] = r < m ? Y[(long)(ib + r) * n + j] : 0.0;
+#pragma unroll
+  for (int r = 0; r < kCB; ++r) {
+    if (r < m) {
+#pragma unroll
+      for (int l = 0; l < r; ++l) x[r] -= Lb[r][l] * x[l];
+      x[r] /= Lb[r][r];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kCB; ++r)
+    if (r < m) Y[(long)(ib + r) * n + j] = x[r];
 }
 
-// Kinv = Y^T Y (symmetric), one thread per entry; Yt = Y^T row-major makes
-// both operands of every dot product contiguous.
-__global__ __launch_bounds__(kBlock) void k_yty(int n, const double* __restrict__ Yt,
-                                                double* __restrict__ K) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)n * n) return;
-  const int i = (int)(e / n), j = (int)(e % n);
-  const int k0 = i > j ? i : j;  // Y[k][i] = 0 for k < i
-  const double* a = Yt + (long)i * n;
-  const double* b = Yt + (long)j * n;
-  double acc = 0.0;
-  for (int k = k0; k < n; ++k) acc += a[k] * b[k];
-  K[e] = acc;
+// tile q = (row tile R of the rows below the block, column tile C <= the
+// block's): Y[R][C] -= L[R][block] * Y[block][C]
+__global__ __launch_bounds__(256) void k_inv_update(int n, const double* __restrict__ L,
+                                                    double* __restrict__ Y, int ib, int ncol) {
+  __shared__ double Lr[kCB][kCB + 1];
+  __shared__ double Yb[kCB][kCB + 1];
+  const int m = min(kCB, n - ib);
+  const int R = blockIdx.x / ncol, C = blockIdx.x % ncol;
+  const int r0 = ib + m + R * kCB, c0 = C * kCB;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kCB * kCB; e += blockDim.x) {
+    const int r = e / kCB, l = e % kCB;
+    Lr[r][l] = (r0 + r < n && l < m) ? L[(long)(r0 + r) * n + ib + l] : 0.0;
+    Yb[r][l] = (r < m && c0 + l < n) ? Y[(long)(ib + r) * n + c0 + l] : 0.0;
+  }
+  __syncthreads();
+  const int rr = (tid / 16) * 4, cc = (tid % 16) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = r0 + rr + a, j = c0 + cc + b;
+      acc[a][b] = (i < n && j < n) ? Y[(long)i * n + j] : 0.0;
+    }
+  for (int l = 0; l < m; ++l) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) av[a] = Lr[rr + a][l];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) bv[b] = Yb[l][cc + b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] -= av[a] * bv[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = r0 + rr + a, j = c0 + cc + b;
+      if (i < n && j < n) Y[(long)i * n + j] = acc[a][b];
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_transpose(int n, const double* __restrict__ A,
-                                                      double* __restrict__ B) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)n * n) return;
-  const int i = (int)(e / n), j = (int)(e % n);
-  B[(long)j * n + i] = A[e];
+// Kinv = Y^T Y over lower 64x64 tiles (I >= J), mirrored into the upper
+// triangle: Kinv[i][j] = sum_{k >= max(i,j)} Y[k][i] Y[k][j], k ascending
+// from +0 (rows k < max(i,j) add exact zeros).
+__global__ __launch_bounds__(256) void k_yty_tiles(int n, const double* __restrict__ Y,
+                                                   double* __restrict__ K) {
+  __shared__ double Ai[kCB][kCB + 1];
+  __shared__ double Aj[kCB][kCB + 1];
+  const int q = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= q) ++ti;
+  while (ti * (ti + 1) / 2 > q) --ti;
+  const int tj = q - ti * (ti + 1) / 2;
+  const int i0 = ti * kCB, j0 = tj * kCB;
+  const int tid = threadIdx.x;
+  const int rr = (tid / 16) * 4, cc = (tid % 16) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int k0 = i0; k0 < n; k0 += kCB) {
+    __syncthreads();
+    for (int e = tid; e < kCB * kCB; e += blockDim.x) {
+      const int l = e / kCB, c = e % kCB;
+      const bool in = k0 + l < n;
+      Ai[l][c] = (in && i0 + c < n) ? Y[(long)(k0 + l) * n + i0 + c] : 0.0;
+      Aj[l][c] = (in && j0 + c < n) ? Y[(long)(k0 + l) * n + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int lm = min(kCB, n - k0);
+    for (int l = 0; l < lm; ++l) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = Ai[l][rr + a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Aj[l][cc + b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = i0 + rr + a, j = j0 + cc + b;
+      if (i < n && j < n) {
+        K[(long)i * n + j] = acc[a][b];
+        K[(long)j * n + i] = acc[a][b];
+      }
+    }
 }
 
-// Kinv = L^{-T} L^{-1} from the lower Cholesky factor L (row-major).  Y and
-// Yt are n*n scratch buffers.
-int dense_inverse_from_chol(int n, const double* L, double* Y, double* Yt, double* Kinv,
+// Kinv = L^{-T} L^{-1} from the lower Cholesky factor L (row-major); Y is an
+// n*n scratch buffer (the caller's second scratch buffer Yt is unused).
+int dense_inverse_from_chol(int n, const double* L, double* Y, double* /*Yt*/, double* Kinv,
                             hipStream_t st) {
-  hipLaunchKernelGGL(k_tri_inverse, dim3(cdiv(n, 64)), dim3(64), 0, st, n, L, Y);
+  const long nn = (long)n * n;
+  hipLaunchKernelGGL(k_eye, dim3(cdiv(nn, kBlock)), dim3(kBlock), 0, st, nn, n, Y);
   GF_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_transpose, dim3(cdiv((long)n * n, kBlock)), dim3(kBlock), 0, st, n, Y, Yt);
-  GF_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_yty, dim3(cdiv((long)n * n, kBlock)), dim3(kBlock), 0, st, n, Yt, Kinv);
+  for (int ib = 0; ib < n; ib += kCB) {
+    const int m = std::min(kCB, n - ib);
+    hipLaunchKernelGGL(k_inv_diag_rows, dim3(cdiv(ib + m, kCB)), dim3(kCB), 0, st, n, L, Y, ib);
+    GF_CHECK_LAUNCH();
+    const int rows = n - ib - m;
+    if (rows <= 0) break;
+    const int ncol = cdiv(ib + m, kCB);
+    hipLaunchKernelGGL(k_inv_update, dim3(cdiv(rows, kCB) * ncol), dim3(256), 0, st, n, L, Y,
+                       ib, ncol);
+    GF_CHECK_LAUNCH();
+  }
+  const int T = cdiv(n, kCB);
+  hipLaunchKernelGGL(k_yty_tiles, dim3(T * (T + 1) / 2), dim3(256), 0, st, n, Y, Kinv);
   GF_CHECK_LAUNCH();
   return 0;
 }

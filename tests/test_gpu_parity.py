@@ -191,6 +191,38 @@ def test_coarsest_pcg_parity(rho_min, tol):
     assert d <= tol
 
 
+@pytest.mark.parametrize("dims,n_free", [((16, 8, 8), 600), ((24, 12, 12), 1764)])
+def test_coarsest_dense_blocked_parity(dims, n_free):
+    """The blocked dense coarsest factorisation (64-wide panels; 600 = 9*64+24
+    and 1764 = 27*64+36 free DOFs, so ragged last panels) and its explicit
+    inverse, against the oracle's dense Cholesky, through a linear two-level
+    cycle (no line search, no Krylov wrappers).  The coarse solve enters once,
+    so the two differ by at most kappa(K_c) * eps relative."""
+    from gpu_fem import policy as gpol
+    from gpu_fem.presets import cantilever
+
+    lin = dict(n_levels=2, dense_chol_max=5000, coarse_correction_policy="none",
+               inner_krylov_steps=0, root_local_correction_mode="none")
+    s = _setup(dims, "tip", rho_min=1e-3, **lin)
+    og = s["og"]
+    og.setup(s["E"])
+    kappa = np.linalg.cond(og._K[-1].toarray())
+    rhs = np.random.default_rng(5).standard_normal(len(s["free"]))
+    zo = og.apply(rhs)
+    for mode in ("inverse", "triangular"):
+        st = dict(STACK)
+        st.update(lin)
+        st["coarsest_solve"] = mode
+        gg = gpol._build_components(cantilever(*dims), **st)[7]
+        gg.setup(s["Et"])
+        assert gg.level_free(1).size == n_free
+        assert gg.coarse_chol_mode == "dense"
+        z = _n(gg.apply(_t(rhs)))
+        err = np.abs(z - zo).max() / np.abs(zo).max()
+        print(mode, "kappa", kappa, "rel err vs oracle", err)
+        assert err <= 100 * kappa * 2.2e-16
+
+
 @pytest.mark.parametrize("dims,seed,p,rho_min,ctol", [((16, 8, 8), 43, 0.35, 1e-12, 1e-6),
                                                       ((16, 8, 8), 41, 0.15, 1e-12, None),
                                                       ((12, 6, 6), 7, 0.5, 1e-3, 1e-6),
