@@ -206,6 +206,41 @@ int32_t gf_gmg_prolong(gf_gmg* g, int32_t level, const double* ec, double* ef, v
 /* free-DOF list of a level (host ints, n_free[level] entries) */
 int32_t gf_gmg_level_free(gf_gmg* g, int32_t level, int32_t* out_host);
 
+/* ---- diagnostics: traced cycle (GalerkinMatFreeGMG.cycle_trace,
+ * multigrid_v4.py:1361-1728) ------------------------------------------- */
+#define GF_TRACE_SLOTS 32
+/* One level visit of a traced cycle, in visit-id (pre-order) order.  v holds
+ * the visit's norms and line-search scalars:
+ *   0 |rhs|  1 |D^-1 rhs|  2 |x after pre-smoothing|  3 |pre-smoothed
+ *   residual r|  4 |R r|  5 |P R r|  6 |r - P R r|  7 |e_c|  8 |P e_c|
+ *   9 r.(P e_c)  10 |residual after the first correction|  11-17 first
+ *   correction scalars [c.c, r.r, Ac.Ac, r.Ac, applied scale, degenerate
+ *   flag, |trial residual|] (line search only)  18 |second coarse rhs|
+ *   19 |e_c2|  20 |P e_c2|  21 r1.(P e_c2)  22 |residual after the second
+ *   correction|  23-29 second correction scalars  30 |x after
+ *   post-smoothing|  31 |rhs - K x_post|.
+ * A coarsest visit fills 0, 1, 7 (|x|) and 10 (|rhs - K x|). */
+typedef struct gf_trace_visit {
+  int32_t visit_id;
+  int32_t level;
+  int32_t is_coarsest;
+  int32_t n_coarse_corrections;
+  int32_t enrichment;            /* second correction from root enrichment */
+  int32_t extra_fine_postsmooth; /* high-contrast second fine post-smooth */
+  double v[GF_TRACE_SLOTS];
+} gf_trace_visit;
+/* One traced cycle on rhs (cycle_type -1 = configured, 0 = V, 1 = W)
+ * followed by the configured root-local correction; out receives the same
+ * iterate as gf_gmg_apply with inner_krylov_steps = 0.  visits: host array
+ * of capacity cap (n_visits receives the count, also when cap is too
+ * small).  summary (host, 16 doubles): 0 |rhs|  1 |D0^-1 rhs|  2 |out|
+ * 3 rhs.out  4 |pre-local residual|  5-8 root-local scalars (inner FGMRES:
+ * [c.c, r.r, t.t]; line search: [c.c, r.r, Ac.Ac, r.Ac])  9 |line-search
+ * trial residual|.  Allocates a small device buffer; synchronises. */
+int32_t gf_gmg_cycle_trace(gf_gmg* g, const double* rhs, double* out, int32_t cycle_type,
+                           gf_trace_visit* visits, int32_t cap, int32_t* n_visits,
+                           double* summary, void* stream);
+
 /* FGMRES(restart) with A = K_ff(E of the last setup), M = gf_gmg_apply.
  * b, x: n_free device.  x0_given: use x as start vector.  history: host
  * array receiving the relative residual history (capacity hist_cap). */

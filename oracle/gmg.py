@@ -240,6 +240,49 @@ def chebyshev_steps(A_op, rhs, x0, d_inv, lam_max, lower_frac, degree):
     return x
 
 
+def lanczos_kappa(op, v, n_iter):
+    """lambda_max / lambda_min of the n_iter-step Lanczos tridiagonal of op
+    from the unit vector v (multigrid_v4.py:1083-1115): absolute breakdown at
+    beta < 1e-14, positive Ritz values only, NaN if there are none."""
+    alpha = np.zeros(n_iter)
+    beta = np.zeros(max(n_iter - 1, 0))
+    v_prev = np.zeros_like(v)
+    for j in range(n_iter):
+        w = op(v)
+        alpha[j] = float(np.dot(v, w))
+        w = w - alpha[j] * v
+        if j > 0:
+            w = w - beta[j - 1] * v_prev
+        if j < n_iter - 1:
+            beta[j] = float(np.linalg.norm(w))
+            if beta[j] < 1e-14:
+                alpha, beta = alpha[: j + 1], beta[:j]
+                break
+            v_prev, v = v, w / beta[j]
+    ev = np.linalg.eigvalsh(np.diag(alpha) + np.diag(beta, 1) + np.diag(beta, -1))
+    ev = ev[ev > 0]
+    return float("nan") if len(ev) == 0 else float(ev[-1] / max(ev[0], 1e-300))
+
+
+def _safe_ratio(num, den):
+    """multigrid_v4.py:840-843."""
+    if abs(den) <= 1e-300:
+        return 0.0 if abs(num) <= 1e-300 else float("inf")
+    return float(num / den)
+
+
+def _merge_notes(*notes):
+    return "|".join(n for n in notes if n)
+
+
+def _alignment(ref, corr):
+    """multigrid_v4.py:855-862."""
+    a, b = float(np.linalg.norm(ref)), float(np.linalg.norm(corr))
+    if a <= 1e-300 or b <= 1e-300:
+        return 0.0
+    return float(np.dot(ref, corr) / (a * b))
+
+
 @dataclass
 class Decision:
     x: object
@@ -502,7 +545,12 @@ class GalerkinMatFreeGMG:
         tn = float(np.linalg.norm(trial))
         if tn > self._cc_ratio * rn:
             return Decision(x, residual, raw, 0.0, "rejected_correction")
-        return Decision(x + s * corr, trial, raw, s)
+        note = ""
+        if s <= 1e-12 and raw <= 0.0:
+            note = "nonpositive_correction_scale"
+        elif abs(s - raw) > 1e-12:
+            note = "clipped_correction_scale"
+        return Decision(x + s * corr, trial, raw, s, note)
 
     def _root_enrichment_rhs(self, residual):
         w = self._re_weights
@@ -546,7 +594,8 @@ class GalerkinMatFreeGMG:
         s = min(self._rl_max, max(0.0, raw))
         if s <= 1e-12:
             return Decision(x, residual, raw, 0.0, "nonpositive_root_local_scale")
-        return Decision(x + s * corr, residual - s * Ac, raw, s)
+        note = "clipped_root_local_scale" if abs(s - raw) > 1e-12 else ""
+        return Decision(x + s * corr, residual - s * Ac, raw, s, note)
 
     # -- apply (multigrid_v4.py:1730-1767) ------------------------------------
     def _apply_base_cycle(self, rhs):
@@ -700,4 +749,160 @@ class GalerkinMatFreeGMG:
         x = self._smooth_fine(x, rhs) if lv == 0 else self._smooth_coarse(lv, x, rhs)
         if lv == 0 and self._is_high_contrast:
             x = self._smooth_fine(x, rhs)
+        return x
+
+    # -- diagnostics (multigrid_v4.py:1057-1117, 1361-1728) -------------------
+    def estimate_kappa_eff(self, n_iter=30, v0=None):
+        """Lanczos on M K (multigrid_v4.py:1057-1117).  Start vector: v0, or
+        numpy default_rng(0) (the reference's CuPy RandomState(0) stream is
+        not available)."""
+        if not self._is_setup:
+            raise RuntimeError("Call setup() before estimate_kappa_eff().")
+        if (self._cc_policy != "none" or self._re_mode != "none" or self._rl_mode != "none"
+                or self._ik_steps > 0):
+            raise RuntimeError(
+                "estimate_kappa_eff requires a linear preconditioner; disable safeguarded "
+                "coarse correction, root enrichment, root-local correction, and inner Krylov "
+                "wrapping first.")
+        n = len(self._free[0])
+        v = np.random.default_rng(0).standard_normal(n) if v0 is None else np.array(v0, float)
+        v = v / float(np.linalg.norm(v))
+        return lanczos_kappa(lambda u: self.apply(self.apply_fine_operator(u)), v, int(n_iter))
+
+    def cycle_trace(self, rhs, cycle_type=None):
+        """One traced cycle plus the root-local correction, as plain dicts with
+        the fields of the reference's MultigridCycleTrace / MultigridCycleVisit
+        (multigrid_v4.py:1361-1728)."""
+        if not self._is_setup:
+            raise RuntimeError("GalerkinMatFreeGMG.setup() must be called before cycle_trace().")
+        rhs = np.asarray(rhs, dtype=np.float64)
+        cycle = self._cycle_type if cycle_type is None else cycle_type
+        if cycle not in {"v", "w"}:
+            raise ValueError(f"cycle_type must be 'v' or 'w'; got {cycle!r}")
+        visits, counter = [], [0]
+        out = self._trace_level(0, rhs, visits, counter, cycle)
+        pre = rhs - self.apply_fine_operator(out)
+        dec = self._apply_root_local_correction(out, pre)
+        out = dec.x
+        nrm = lambda u: float(np.linalg.norm(u))
+        rn, jn, on, resn = nrm(rhs), nrm(self._d_inv[0] * rhs), nrm(out), nrm(dec.residual)
+        rz = float(np.dot(rhs, out))
+        visits.sort(key=lambda t: t["visit_id"])
+        if dec.note:
+            root = next((t for t in visits if t["level"] == 0 and not t["is_coarsest"]), None)
+            if root is not None:
+                root["notes"] = _merge_notes(root["notes"], dec.note)
+        ap = self.adaptive_params
+        return dict(
+            cycle_type=cycle, rhs_norm=rn, jacobi_action_norm=jn, output_norm=on,
+            z_over_jacobi=_safe_ratio(on, jn), rz=rz, pd=bool(rz > 0.0),
+            output_residual_norm=resn, output_residual_ratio=_safe_ratio(resn, rn),
+            contrast_ratio=self._contrast_ratio, is_high_contrast=self._is_high_contrast,
+            fine_degree_adapt=self._fine_degree_adapt, coarse_iters_adapt=self._coarse_iters_adapt,
+            coarse_correction_policy=self._cc_policy,
+            coarse_correction_acceptance_ratio=self._cc_ratio,
+            coarse_correction_max_scale=self._cc_max, root_enrichment_mode=self._re_mode,
+            root_enrichment_weight_floor=self._re_floor,
+            root_enrichment_weight_power=self._re_power,
+            root_local_correction_mode=self._rl_mode,
+            root_local_correction_inner_steps=self._rl_steps,
+            root_local_correction_inner_restart=self._rl_restart,
+            root_local_correction_raw_scale=dec.raw_scale,
+            root_local_correction_applied_scale=dec.applied_scale,
+            root_local_correction_residual_ratio=_safe_ratio(resn, nrm(pre)),
+            inner_krylov_steps=self._ik_steps, inner_krylov_restart=self._ik_restart,
+            visits=visits, output=out)
+
+    def _trace_level(self, lv, rhs, visits, counter, cycle):
+        """_trace_vcycle_level / _trace_wcycle_level (multigrid_v4.py:1438-1728)."""
+        nrm = lambda u: float(np.linalg.norm(u))
+        vid = counter[0]
+        counter[0] += 1
+        rn = nrm(rhs)
+        jn = nrm(self._d_inv[lv] * rhs)
+        if lv == self._n_levels - 1:
+            x = self._solve_coarsest(rhs)
+            res = nrm(rhs - self._level_matvec(lv, x))
+            xn = nrm(x)
+            visits.append(dict(
+                visit_id=vid, level=lv, cycle_type=cycle, is_coarsest=True, n_coarse_corrections=0,
+                rhs_norm=rn, jacobi_action_norm=jn, presmooth_solution_norm=0.0,
+                presmooth_residual_norm=rn, presmooth_residual_ratio=1.0, restricted_rhs_norm=0.0,
+                restricted_rhs_ratio=0.0, projector_norm=0.0, coarse_space_gap_norm=0.0,
+                coarse_space_gap_ratio=0.0, coarse_correction_norm=xn,
+                prolongated_correction_norm=xn, correction_alignment=0.0,
+                corrected_residual_norm=res, corrected_residual_vs_presmooth=_safe_ratio(res, rn),
+                correction_scale_raw=None, correction_scale_applied=1.0,
+                second_restricted_rhs_norm=None, second_coarse_correction_norm=None,
+                second_prolongated_correction_norm=None, second_correction_alignment=None,
+                second_correction_scale_raw=None, second_correction_scale_applied=None,
+                second_corrected_residual_norm=None, second_corrected_residual_vs_corrected=None,
+                postsmooth_solution_norm=xn, postsmooth_residual_norm=res,
+                postsmooth_residual_ratio=_safe_ratio(res, rn), coarsest_mode=self._coarse_chol_mode,
+                notes=""))
+            return x
+        z = np.zeros_like(rhs)
+        xs = self._smooth_fine(z, rhs) if lv == 0 else self._smooth_coarse(lv, z, rhs)
+        r = rhs - self._level_matvec(lv, xs)
+        prn = nrm(r)
+        rc = self._R[lv] @ r
+        proj = self._P[lv] @ rc
+        ec = self._trace_level(lv + 1, rc, visits, counter, cycle)
+        Pe = self._P[lv] @ ec
+        d1 = self._apply_coarse_correction(lv, xs, r, Pe)
+        x, r1 = d1.x, d1.residual
+        c1n = nrm(r1)
+        second = {}
+        d2 = None
+        enote = ""
+        corrected_n = c1n
+        if cycle == "w" or (lv == 0 and self._re_mode != "none"):
+            if lv == 0 and self._re_mode != "none":
+                rc2 = self._root_enrichment_rhs(r1)
+                enote = "root_enrichment:" + self._re_mode
+            else:
+                rc2 = self._R[lv] @ r1
+            e2 = self._trace_level(lv + 1, rc2, visits, counter, cycle)
+            Pe2 = self._P[lv] @ e2
+            d2 = self._apply_coarse_correction(lv, x, r1, Pe2)
+            x = d2.x
+            c2n = nrm(d2.residual)
+            if cycle == "v":
+                corrected_n = c2n
+            second = dict(second_restricted_rhs_norm=nrm(rc2), second_coarse_correction_norm=nrm(e2),
+                          second_prolongated_correction_norm=nrm(Pe2),
+                          second_correction_alignment=_alignment(r1, Pe2),
+                          second_correction_scale_raw=d2.raw_scale,
+                          second_correction_scale_applied=d2.applied_scale,
+                          second_corrected_residual_norm=c2n,
+                          second_corrected_residual_vs_corrected=_safe_ratio(c2n, c1n))
+        else:
+            second = dict(second_restricted_rhs_norm=None, second_coarse_correction_norm=None,
+                          second_prolongated_correction_norm=None, second_correction_alignment=None,
+                          second_correction_scale_raw=None, second_correction_scale_applied=None,
+                          second_corrected_residual_norm=None,
+                          second_corrected_residual_vs_corrected=None)
+        extra = ""
+        if lv == 0:
+            x = self._smooth_fine(x, rhs)
+            if self._is_high_contrast:
+                x = self._smooth_fine(x, rhs)
+                extra = "extra_fine_postsmooth"
+        else:
+            x = self._smooth_coarse(lv, x, rhs)
+        post = nrm(rhs - self._level_matvec(lv, x))
+        visits.append(dict(
+            visit_id=vid, level=lv, cycle_type=cycle, is_coarsest=False,
+            n_coarse_corrections=2 if d2 is not None else 1, rhs_norm=rn, jacobi_action_norm=jn,
+            presmooth_solution_norm=nrm(xs), presmooth_residual_norm=prn,
+            presmooth_residual_ratio=_safe_ratio(prn, rn), restricted_rhs_norm=nrm(rc),
+            restricted_rhs_ratio=_safe_ratio(nrm(rc), prn), projector_norm=nrm(proj),
+            coarse_space_gap_norm=nrm(r - proj), coarse_space_gap_ratio=_safe_ratio(nrm(r - proj), prn),
+            coarse_correction_norm=nrm(ec), prolongated_correction_norm=nrm(Pe),
+            correction_alignment=_alignment(r, Pe), corrected_residual_norm=corrected_n,
+            corrected_residual_vs_presmooth=_safe_ratio(corrected_n, prn),
+            correction_scale_raw=d1.raw_scale, correction_scale_applied=d1.applied_scale,
+            postsmooth_solution_norm=nrm(x), postsmooth_residual_norm=post,
+            postsmooth_residual_ratio=_safe_ratio(post, rn), coarsest_mode="",
+            notes=_merge_notes(d1.note, d2.note if d2 is not None else "", enote, extra), **second))
         return x

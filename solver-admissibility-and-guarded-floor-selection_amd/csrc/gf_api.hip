@@ -337,6 +337,16 @@ int32_t gf_gmg_vcycle(gf_gmg* g, const double* rhs, double* out, void* stream) {
   return g->g.base_cycle(rhs, out, nullptr, S(stream));
 }
 
+int32_t gf_gmg_cycle_trace(gf_gmg* g, const double* rhs, double* out, int32_t cycle_type,
+                           gf_trace_visit* visits, int32_t cap, int32_t* n_visits,
+                           double* summary, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(rhs);
+  CHECK_PTR(out);
+  CHECK_PTR(visits);
+  return g->g.cycle_trace(rhs, out, cycle_type, visits, cap, n_visits, summary, S(stream));
+}
+
 static int check_level(gf_gmg* g, int level, bool need_coarse) {
   if (!g->g.is_setup) {
     set_error("GalerkinMatFreeGMG.setup() must be called first");

@@ -139,6 +139,21 @@ struct Gmg {
   int setup(const double* E, const double* fine_diag, hipStream_t st);
   int apply(const double* rhs, double* out, hipStream_t st);
   int apply_impl(const double* rhs, double* out, hipStream_t st);
+  int root_local(const double* rhs, const double* xp, double* out, hipStream_t st);
+  // traced cycle (diagnostics): slots of one visit are GF_TRACE_SLOTS doubles
+  struct TraceMeta {
+    int level, is_coarsest, n_corr, enrich, extra;
+  };
+  struct TraceState {
+    std::vector<TraceMeta> meta;
+    double* slots = nullptr;  // device, cap * GF_TRACE_SLOTS
+    const double* one = nullptr;
+    int cap = 0;
+  };
+  int trace_visit(int l, const double* rhs, double* out, bool wcyc, hipStream_t st,
+                  double** result, TraceState& ts);
+  int cycle_trace(const double* rhs, double* out, int cycle_type, gf_trace_visit* visits, int cap,
+                  int* n_visits, double* summary, hipStream_t st);
   int capture();
   void drop_graph();
   int base_cycle(const double* rhs, double* out, const int* gate, hipStream_t st);

@@ -1080,6 +1080,16 @@ int Gmg::apply_impl(const double* rhs, double* out, hipStream_t st) {
     LAUNCH(k_copy, n, st, n, xp, out, (const int*)nullptr);
     return 0;
   }
+  return root_local(rhs, xp, out, st);
+}
+
+// Root-local correction of x_p against rhs (multigrid_v4.py:964-1055): the
+// pre-local residual goes to ap_res, the decision scalars to ap_st, the
+// corrected iterate to out.  Inner FGMRES mode: ap_st = [c.c, r.r, t.t] with
+// the trial residual t in ap_trial; line-search mode: ap_st = [c.c, r.r,
+// Ac.Ac, r.Ac] with Ac in ap_trial.
+int Gmg::root_local(const double* rhs, const double* xp, double* out, hipStream_t st) {
+  const long n = lv[0].n_free;
   EpiArgs<double> ep;
   ep.b = rhs;
   GF_TRY(level_op(0, xp, ap_res, EPI_RESID, ep, nullptr, st));
@@ -1415,6 +1425,244 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
   *converged = conv ? 1 : 0;
   *n_hist = nh;
   return 0;
+}
+
+}  // namespace gf
+
+namespace gf {
+
+// ---------------------------------------------------------------------------
+// traced cycle (multigrid_v4.py:1361-1728): one V or W cycle on rhs with the
+// kernels of cycle() in the same order (the iterate, and so the output, is
+// the one cycle() produces), plus the norms and dot products of the
+// reference's per-visit record, written to device slots and read back once.
+// Slot layout of one visit (gf_trace_visit.v):
+//   0 |rhs|  1 |D^-1 rhs|  2 |x_pre|  3 |r_pre|  4 |R r_pre|  5 |P R r_pre|
+//   6 |r_pre - P R r_pre|  7 |e_c|  8 |P e_c|  9 r_pre.(P e_c)
+//   10 |r after the first correction|  11-17 first line-search scalars
+//      [c.c, r.r, Ac.Ac, r.Ac, applied scale, degenerate, |trial|]
+//   18 |second coarse rhs|  19 |e_c2|  20 |P e_c2|  21 r_1.(P e_c2)
+//   22 |r after the second correction|  23-29 second line-search scalars
+//   30 |x_post|  31 |rhs - K x_post|
+// Coarsest visit: 0, 1, 7 = |x|, 10 = |rhs - K x|.
+// Summary (16 doubles): 0 |rhs|  1 |D0^-1 rhs|  2 |out|  3 rhs.out
+//   4 |pre-local residual|  5-8 root-local scalars (ap_st)  9 |line-search
+//   trial residual|.
+// ---------------------------------------------------------------------------
+namespace {
+enum : int {
+  TV_RHS = 0, TV_JAC = 1, TV_PRE_X = 2, TV_PRE_R = 3, TV_RC = 4, TV_PROJ = 5, TV_GAP = 6,
+  TV_EC = 7, TV_PEC = 8, TV_R_PEC = 9, TV_CORR_R = 10, TV_ST1 = 11, TV_RC2 = 18, TV_EC2 = 19,
+  TV_PEC2 = 20, TV_R_PEC2 = 21, TV_CORR_R2 = 22, TV_ST2 = 23, TV_POST_X = 30, TV_POST_R = 31
+};
+static_assert(TV_POST_R < GF_TRACE_SLOTS, "trace slots");
+}  // namespace
+
+// applied root-local line-search scale (multigrid_v4.py:1035-1045)
+__global__ void k_rl_scale(const double* st, double max_scale, double* s_out) {
+  const double cn = sqrt(st[0]), rn = sqrt(st[1]), den = st[2];
+  double s = 0.0;
+  if (!(cn <= 1e-300 || rn <= 1e-300 || den <= 1e-300)) {
+    s = fmin(max_scale, fmax(0.0, st[3] / den));
+    if (s <= 1e-12) s = 0.0;
+  }
+  *s_out = s;
+}
+
+static int trace_nrm(Work& w, long n, const double* x, double* out, hipStream_t st) {
+  return dot_f64(w, n, x, x, out, true, nullptr, st);
+}
+
+int Gmg::trace_visit(int l, const double* rhs, double* out, bool wcyc, hipStream_t st,
+                     double** result, TraceState& ts) {
+  const int id = (int)ts.meta.size();
+  if (id >= ts.cap) {
+    set_error("cycle_trace: more visits than slots");
+    return -2;
+  }
+  ts.meta.push_back(TraceMeta{l, l == L - 1 ? 1 : 0, 0, 0, 0});
+  double* s = ts.slots + (long)id * GF_TRACE_SLOTS;
+  Level& v = lv[l];
+  const long n = v.n_free;
+  GF_TRY(trace_nrm(work, n, rhs, s + TV_RHS, st));
+  LAUNCH(k_mul, n, st, n, v.dinv, rhs, v.T);
+  GF_TRY(trace_nrm(work, n, v.T, s + TV_JAC, st));
+  if (l == L - 1) {
+    double* x = nullptr;
+    GF_TRY(solve_coarsest(rhs, out, nullptr, st, &x));
+    EpiArgs<double> ep;
+    ep.b = rhs;
+    GF_TRY(level_op(l, x, v.T, EPI_RESID, ep, nullptr, st));
+    GF_TRY(trace_nrm(work, n, x, s + TV_EC, st));
+    GF_TRY(trace_nrm(work, n, v.T, s + TV_CORR_R, st));
+    *result = x;
+    return 0;
+  }
+  Level& c = lv[l + 1];
+  const long nc = c.n_free;
+  const bool enrich = (l == 0 && p.root_enrichment_mode != 0);
+  const bool ls = p.coarse_correction_policy != 0;
+  double* x = nullptr;
+  GF_TRY(smooth(l, nullptr, rhs, nullptr, 1, nullptr, st, &x));
+  {
+    EpiArgs<double> ep;
+    ep.b = rhs;
+    GF_TRY(level_op(l, x, v.R, EPI_RESID, ep, nullptr, st));
+  }
+  GF_TRY(trace_nrm(work, n, x, s + TV_PRE_X, st));
+  GF_TRY(trace_nrm(work, n, v.R, s + TV_PRE_R, st));
+  GF_TRY(restrict_(v.g, v.map(), c.g, c.map(), v.R, c.RHS, nullptr, st));
+  GF_TRY(trace_nrm(work, nc, c.RHS, s + TV_RC, st));
+  GF_TRY(prolong(v.g, v.map(), c.g, c.map(), c.RHS, v.T, nullptr, st));
+  GF_TRY(trace_nrm(work, n, v.T, s + TV_PROJ, st));
+  GF_TRY(trial_norm(work, n, v.R, v.T, ts.one, nullptr, s + TV_GAP, nullptr, st));
+  double* ec = nullptr;
+  GF_TRY(trace_visit(l + 1, c.RHS, nullptr, wcyc, st, &ec, ts));
+  GF_TRY(trace_nrm(work, nc, ec, s + TV_EC, st));
+  GF_TRY(prolong(v.g, v.map(), c.g, c.map(), ec, v.PE, nullptr, st));
+  GF_TRY(trace_nrm(work, n, v.PE, s + TV_PEC, st));
+  GF_TRY(dot_f64(work, n, v.R, v.PE, s + TV_R_PEC, false, nullptr, st));
+  const size_t st_bytes = 7 * sizeof(double);
+  int n_corr = 1;
+  bool second = false;
+  if (!wcyc) {
+    GF_TRY(correct(l, x, v.R, v.PE, v.R2, nullptr, st));
+    if (ls) GF_HIP(hipMemcpyAsync(s + TV_ST1, v.st, st_bytes, hipMemcpyDeviceToDevice, st));
+    GF_TRY(trace_nrm(work, n, v.R2, s + TV_CORR_R, st));
+    if (enrich) {
+      GF_TRY(enrichment_rhs(v.R2, c.RHS, nullptr, st));
+      second = true;
+    }
+  } else {
+    if (!ls) {
+      LAUNCH(k_add, n, st, n, x, v.PE, (const int*)nullptr);
+      EpiArgs<double> ep;
+      ep.b = rhs;
+      GF_TRY(level_op(l, x, v.R2, EPI_RESID, ep, nullptr, st));
+    } else {
+      GF_TRY(correct(l, x, v.R, v.PE, v.R2, nullptr, st));
+      GF_HIP(hipMemcpyAsync(s + TV_ST1, v.st, st_bytes, hipMemcpyDeviceToDevice, st));
+    }
+    GF_TRY(trace_nrm(work, n, v.R2, s + TV_CORR_R, st));
+    if (enrich)
+      GF_TRY(enrichment_rhs(v.R2, c.RHS, nullptr, st));
+    else
+      GF_TRY(restrict_(v.g, v.map(), c.g, c.map(), v.R2, c.RHS, nullptr, st));
+    second = true;
+  }
+  if (second) {
+    GF_TRY(trace_nrm(work, nc, c.RHS, s + TV_RC2, st));
+    double* e2 = nullptr;
+    GF_TRY(trace_visit(l + 1, c.RHS, nullptr, wcyc, st, &e2, ts));
+    GF_TRY(trace_nrm(work, nc, e2, s + TV_EC2, st));
+    GF_TRY(prolong(v.g, v.map(), c.g, c.map(), e2, v.PE, nullptr, st));
+    GF_TRY(trace_nrm(work, n, v.PE, s + TV_PEC2, st));
+    GF_TRY(dot_f64(work, n, v.R2, v.PE, s + TV_R_PEC2, false, nullptr, st));
+    GF_TRY(correct(l, x, v.R2, v.PE, v.T, nullptr, st));
+    if (ls) GF_HIP(hipMemcpyAsync(s + TV_ST2, v.st, st_bytes, hipMemcpyDeviceToDevice, st));
+    GF_TRY(trace_nrm(work, n, v.T, s + TV_CORR_R2, st));
+    n_corr = 2;
+  }
+  const bool extra = (l == 0 && high);
+  GF_TRY(smooth(l, x, rhs, extra ? nullptr : out, 1, nullptr, st, &x));
+  if (extra) GF_TRY(smooth(l, x, rhs, out, 1, nullptr, st, &x));
+  {
+    EpiArgs<double> ep;
+    ep.b = rhs;
+    GF_TRY(level_op(l, x, v.T, EPI_RESID, ep, nullptr, st));
+  }
+  GF_TRY(trace_nrm(work, n, x, s + TV_POST_X, st));
+  GF_TRY(trace_nrm(work, n, v.T, s + TV_POST_R, st));
+  TraceMeta& m = ts.meta[id];
+  m.n_corr = n_corr;
+  m.enrich = enrich ? 1 : 0;
+  m.extra = extra ? 1 : 0;
+  *result = x;
+  return 0;
+}
+
+static int trace_visit_count(int l, int L, bool wcyc, bool enrich) {
+  if (l == L - 1) return 1;
+  const int k = (wcyc || (l == 0 && enrich)) ? 2 : 1;
+  return 1 + k * trace_visit_count(l + 1, L, wcyc, enrich);
+}
+
+int Gmg::cycle_trace(const double* rhs, double* out, int cycle_type, gf_trace_visit* visits,
+                     int cap, int* n_visits, double* summary, hipStream_t st) {
+  if (!is_setup) {
+    set_error("GalerkinMatFreeGMG.setup() must be called before cycle_trace().");
+    return -3;
+  }
+  if (cycle_type < -1 || cycle_type > 1) {
+    set_error("cycle_trace: cycle_type must be -1 (configured), 0 (V) or 1 (W)");
+    return -2;
+  }
+  const bool wcyc = cycle_type < 0 ? p.cycle_type == 1 : cycle_type == 1;
+  const int nv = trace_visit_count(0, L, wcyc, p.root_enrichment_mode != 0);
+  if (n_visits) *n_visits = nv;
+  if (nv > cap) {
+    set_error("cycle_trace: visit capacity too small");
+    return -2;
+  }
+  const long n = lv[0].n_free;
+  const long nslots = (long)nv * GF_TRACE_SLOTS + 16 + 1;
+  double* dbuf = nullptr;
+  GF_TRY(dalloc(&dbuf, nslots));
+  TraceState ts;
+  ts.slots = dbuf;
+  ts.cap = nv;
+  double* sm = dbuf + (long)nv * GF_TRACE_SLOTS;
+  double* one = sm + 16;
+  ts.one = one;
+  int rc = 0;
+  do {
+    LAUNCH(k_fill, nslots, st, nslots, dbuf, 0.0, (const int*)nullptr);
+    LAUNCH(k_fill, 1, st, 1, one, 1.0, (const int*)nullptr);
+    const int rl = p.root_local_correction_mode;
+    double* target = rl == 0 ? out : ap_x;
+    double* xr = nullptr;
+    if ((rc = trace_visit(0, rhs, target, wcyc, st, &xr, ts))) break;
+    if (xr != target) LAUNCH(k_copy, n, st, n, xr, target, (const int*)nullptr);
+    if (rl == 0) {
+      EpiArgs<double> ep;
+      ep.b = rhs;
+      if ((rc = level_op(0, out, ap_res, EPI_RESID, ep, nullptr, st))) break;
+    } else {
+      if ((rc = root_local(rhs, ap_x, out, st))) break;
+      GF_HIP(hipMemcpyAsync(sm + 5, ap_st, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
+      if (rl == 1) {
+        hipLaunchKernelGGL(k_rl_scale, dim3(1), dim3(1), 0, st, ap_st,
+                           p.root_local_correction_max_scale, sm + 10);
+        GF_CHECK_LAUNCH();
+        if ((rc = trial_norm(work, n, ap_res, ap_trial, sm + 10, nullptr, sm + 9, nullptr, st)))
+          break;
+      }
+    }
+    if ((rc = trace_nrm(work, n, ap_res, sm + 4, st))) break;
+    if ((rc = trace_nrm(work, n, rhs, sm + 0, st))) break;
+    LAUNCH(k_mul, n, st, n, lv[0].dinv, rhs, lv[0].T);
+    if ((rc = trace_nrm(work, n, lv[0].T, sm + 1, st))) break;
+    if ((rc = trace_nrm(work, n, out, sm + 2, st))) break;
+    if ((rc = dot_f64(work, n, rhs, out, sm + 3, false, nullptr, st))) break;
+    std::vector<double> h(nslots);
+    GF_HIP(hipMemcpyAsync(h.data(), dbuf, sizeof(double) * nslots, hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nv; ++i) {
+      const TraceMeta& m = ts.meta[i];
+      gf_trace_visit& o = visits[i];
+      o.visit_id = i;
+      o.level = m.level;
+      o.is_coarsest = m.is_coarsest;
+      o.n_coarse_corrections = m.n_corr;
+      o.enrichment = m.enrich;
+      o.extra_fine_postsmooth = m.extra;
+      std::memcpy(o.v, h.data() + (long)i * GF_TRACE_SLOTS, sizeof(double) * GF_TRACE_SLOTS);
+    }
+    if (summary) std::memcpy(summary, h.data() + (long)nv * GF_TRACE_SLOTS, sizeof(double) * 16);
+  } while (0);
+  (void)hipStreamSynchronize(st);
+  dfree(dbuf);
+  return rc;
 }
 
 }  // namespace gf

@@ -71,6 +71,21 @@ class gf_gmg_info(C.Structure):
     ]
 
 
+TRACE_SLOTS = 32
+
+
+class gf_trace_visit(C.Structure):
+    _fields_ = [
+        ("visit_id", C.c_int32),
+        ("level", C.c_int32),
+        ("is_coarsest", C.c_int32),
+        ("n_coarse_corrections", C.c_int32),
+        ("enrichment", C.c_int32),
+        ("extra_fine_postsmooth", C.c_int32),
+        ("v", C.c_double * TRACE_SLOTS),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _D = C.c_double
@@ -109,6 +124,7 @@ SIGNATURES = {
     "gf_gmg_restrict": [_P, _I, _P, _P, _P],
     "gf_gmg_prolong": [_P, _I, _P, _P, _P],
     "gf_gmg_level_free": [_P, _I, _P],
+    "gf_gmg_cycle_trace": [_P, _P, _P, _I, C.POINTER(gf_trace_visit), _I, C.POINTER(_I), _P, _P],
     "gf_gmg_solve": [_P, _P, _P, _I, _D, _I, _I, _P, _I, C.POINTER(_I), C.POINTER(_I),
                      C.POINTER(_I), _P],
     "gf_cnn_sizes": [C.POINTER(_I), C.POINTER(C.c_int64)],

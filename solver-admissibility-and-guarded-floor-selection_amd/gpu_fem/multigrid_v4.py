@@ -15,7 +15,8 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from dataclasses import dataclass
+from dataclasses import asdict, dataclass, field
+from typing import Optional
 
 import numpy as np
 
@@ -37,6 +38,139 @@ class MultigridLevelStats:
     n_free: int
     nnz: int
     estimated_vram_bytes: int
+
+
+@dataclass
+class MultigridCycleVisit:
+    """One level visit of a traced cycle (multigrid_v4.py:39-77; same fields)."""
+    visit_id: int
+    level: int
+    cycle_type: str
+    is_coarsest: bool
+    n_coarse_corrections: int
+    rhs_norm: float
+    jacobi_action_norm: float
+    presmooth_solution_norm: float
+    presmooth_residual_norm: float
+    presmooth_residual_ratio: float
+    restricted_rhs_norm: float
+    restricted_rhs_ratio: float
+    projector_norm: float
+    coarse_space_gap_norm: float
+    coarse_space_gap_ratio: float
+    coarse_correction_norm: float
+    prolongated_correction_norm: float
+    correction_alignment: float
+    corrected_residual_norm: float
+    corrected_residual_vs_presmooth: float
+    correction_scale_raw: Optional[float] = None
+    correction_scale_applied: float = 1.0
+    second_restricted_rhs_norm: Optional[float] = None
+    second_coarse_correction_norm: Optional[float] = None
+    second_prolongated_correction_norm: Optional[float] = None
+    second_correction_alignment: Optional[float] = None
+    second_correction_scale_raw: Optional[float] = None
+    second_correction_scale_applied: Optional[float] = None
+    second_corrected_residual_norm: Optional[float] = None
+    second_corrected_residual_vs_corrected: Optional[float] = None
+    postsmooth_solution_norm: float = 0.0
+    postsmooth_residual_norm: float = 0.0
+    postsmooth_residual_ratio: float = 0.0
+    coarsest_mode: str = ""
+    notes: str = ""
+
+    def to_dict(self) -> dict:
+        return asdict(self)
+
+
+@dataclass
+class MultigridCycleTrace:
+    """A traced cycle plus the root-local correction (multigrid_v4.py:80-112)."""
+    cycle_type: str
+    rhs_norm: float
+    jacobi_action_norm: float
+    output_norm: float
+    z_over_jacobi: float
+    rz: float
+    pd: bool
+    output_residual_norm: float
+    output_residual_ratio: float
+    contrast_ratio: float
+    is_high_contrast: bool
+    fine_degree_adapt: int
+    coarse_iters_adapt: int
+    coarse_correction_policy: str
+    coarse_correction_acceptance_ratio: float
+    coarse_correction_max_scale: float
+    root_enrichment_mode: str
+    root_enrichment_weight_floor: float
+    root_enrichment_weight_power: float
+    root_local_correction_mode: str
+    root_local_correction_inner_steps: int
+    root_local_correction_inner_restart: int
+    root_local_correction_raw_scale: Optional[float]
+    root_local_correction_applied_scale: float
+    root_local_correction_residual_ratio: float
+    inner_krylov_steps: int
+    inner_krylov_restart: int
+    visits: list = field(default_factory=list)
+
+    def to_dict(self) -> dict:
+        return asdict(self)
+
+
+def _safe_ratio(num: float, den: float) -> float:
+    """multigrid_v4.py:840-843."""
+    if abs(den) <= 1e-300:
+        return 0.0 if abs(num) <= 1e-300 else float("inf")
+    return float(num / den)
+
+
+def _merge_notes(*notes: str) -> str:
+    """multigrid_v4.py:934-936."""
+    return "|".join(n for n in notes if n)
+
+
+def _alignment(ref_norm: float, corr_norm: float, dot_rc: float) -> float:
+    """Cosine of the angle between a residual and a correction
+    (multigrid_v4.py:855-862)."""
+    if ref_norm <= 1e-300 or corr_norm <= 1e-300:
+        return 0.0
+    return float(dot_rc / (ref_norm * corr_norm))
+
+
+def lanczos_kappa(op, v, n_iter: int, dot_fn, nrm_fn, axpby_fn, zeros_like):
+    """lambda_max / lambda_min of the Lanczos tridiagonal of ``op`` started
+    from the unit vector ``v`` (multigrid_v4.py:1083-1115): n_iter steps,
+    absolute breakdown at beta < 1e-14, positive Ritz values only; NaN if
+    there are none.  The vector arithmetic goes through the callables so the
+    recurrence runs on device vectors with the library's own kernels."""
+    alpha = np.zeros(n_iter, dtype=np.float64)
+    beta = np.zeros(max(n_iter - 1, 0), dtype=np.float64)
+    v_prev = zeros_like(v)
+    for j in range(n_iter):
+        w = op(v)
+        a = dot_fn(v, w)
+        alpha[j] = a
+        axpby_fn(-a, v, 1.0, w)
+        if j > 0:
+            axpby_fn(-beta[j - 1], v_prev, 1.0, w)
+        if j < n_iter - 1:
+            b = nrm_fn(w)
+            beta[j] = b
+            if b < 1e-14:
+                alpha = alpha[: j + 1]
+                beta = beta[:j]
+                break
+            v_prev = v
+            v = zeros_like(w)
+            axpby_fn(1.0 / b, w, 0.0, v)
+    T = np.diag(alpha) + np.diag(beta, 1) + np.diag(beta, -1)
+    eigs = np.linalg.eigvalsh(T)
+    eigs = eigs[eigs > 0]
+    if len(eigs) == 0:
+        return float("nan")
+    return float(eigs[-1] / max(eigs[0], 1e-300))
 
 
 class GalerkinMatFreeGMG:
@@ -338,6 +472,193 @@ class GalerkinMatFreeGMG:
         zn, jn = nrm2(z), nrm2(zj)
         return {"finite": bool(torch.isfinite(z).all().item()), "pd": dot(_as_dev(rhs, torch.float64), z) > 0.0,
                 "z_norm": zn, "jacobi_norm": jn, "z_over_jacobi": zn / max(jn, 1e-30)}
+
+    # ----------------------------------------------------------- diagnostics
+    def estimate_kappa_eff(self, n_iter: int = 30, v0=None) -> float:
+        """kappa_eff = lambda_max / lambda_min of the Lanczos tridiagonal of the
+        applied operator M K (multigrid_v4.py:1057-1117), n_iter steps.  The
+        start vector is v0 (normalised) or, by default, a standard normal
+        vector from numpy's default_rng(0); the reference draws it from
+        CuPy's RandomState(0), which is not available here, so the two start
+        from different vectors (the extreme Ritz values, and so the
+        estimate, depend on the start vector only weakly)."""
+        import torch
+
+        if not self._is_setup:
+            raise RuntimeError("Call setup() before estimate_kappa_eff().")
+        if (self._coarse_correction_policy != "none" or self._root_enrichment_mode != "none"
+                or self._root_local_correction_mode != "none" or self._inner_krylov_steps > 0):
+            raise RuntimeError(
+                "estimate_kappa_eff requires a linear preconditioner; disable safeguarded "
+                "coarse correction, root enrichment, root-local correction, and inner Krylov "
+                "wrapping first.")
+        n = self._n_free[0]
+        if v0 is None:
+            v0 = np.random.default_rng(0).standard_normal(n)
+        v = _as_dev(v0, torch.float64).clone()
+        axpby(1.0 / nrm2(v), v, 0.0, v)
+        self._kappa_eff = lanczos_kappa(self.apply_preconditioned_operator, v, int(n_iter), dot,
+                                        nrm2, axpby, torch.zeros_like)
+        return self._kappa_eff
+
+    def _coarse_decision(self, stv):
+        """(raw, applied, note) of a safeguarded coarse correction from its
+        device scalars [c.c, r.r, Ac.Ac, r.Ac, applied, degenerate, |trial|]
+        (multigrid_v4.py:867-931)."""
+        if self._coarse_correction_policy == "none":
+            return 1.0, 1.0, ""
+        p = self._params
+        rr, den, rac, s, degen, tn = stv[1], stv[2], stv[3], stv[4], stv[5], stv[6]
+        if degen != 0.0:
+            return 0.0, 0.0, "degenerate_correction"
+        raw = float(rac / den)
+        if tn > p.coarse_correction_acceptance_ratio * math.sqrt(rr):
+            return raw, 0.0, "rejected_correction"
+        note = ""
+        if s <= 1e-12 and raw <= 0.0:
+            note = "nonpositive_correction_scale"
+        elif abs(s - raw) > 1e-12:
+            note = "clipped_correction_scale"
+        return raw, float(s), note
+
+    def _root_local_decision(self, sm):
+        """(raw, applied, note, |output residual|) of the root-local
+        correction from the trace summary (multigrid_v4.py:964-1055)."""
+        pre = sm[4]
+        mode = self._root_local_correction_mode
+        if mode == "none":
+            return None, 0.0, "", pre
+        cn, rn = math.sqrt(sm[5]), math.sqrt(sm[6])
+        if mode == "node_block_inner_fgmres":
+            if cn <= 1e-300 or rn <= 1e-300:
+                return 0.0, 0.0, "degenerate_root_local_correction", pre
+            tn = math.sqrt(sm[7])
+            if tn > rn:
+                return None, 0.0, "rejected_root_local_inner_fgmres", pre
+            return None, 1.0, "root_local_inner_fgmres", tn
+        den, rac = sm[7], sm[8]
+        if cn <= 1e-300 or rn <= 1e-300 or den <= 1e-300:
+            return 0.0, 0.0, "degenerate_root_local_correction", pre
+        raw = float(rac / den)
+        s = min(float(self._params.root_local_correction_max_scale), max(0.0, raw))
+        if s <= 1e-12:
+            return raw, 0.0, "nonpositive_root_local_scale", pre
+        note = "clipped_root_local_scale" if abs(s - raw) > 1e-12 else ""
+        return raw, s, note, sm[9]
+
+    def cycle_trace(self, rhs, *, cycle_type: Optional[str] = None) -> MultigridCycleTrace:
+        """One traced V or W cycle on rhs followed by the root-local correction
+        (multigrid_v4.py:1361-1728).  The cycle runs in libgfhip.so with the
+        kernels of the ordinary cycle, so its output is the iterate that
+        apply() returns when inner_krylov_steps = 0; the per-visit norms are
+        reduced on the device and read back once."""
+        import torch
+
+        if not self._is_setup:
+            raise RuntimeError("GalerkinMatFreeGMG.setup() must be called before cycle_trace().")
+        cycle = ("w" if self._params.cycle_type == 1 else "v") if cycle_type is None else cycle_type
+        if cycle not in {"v", "w"}:
+            raise ValueError(f"cycle_type must be 'v' or 'w'; got {cycle!r}")
+        r = _as_dev(rhs, torch.float64)
+        out = torch.empty_like(r)
+        cap = 1
+        while True:
+            visits = (_lib.gf_trace_visit * cap)()
+            nv = C.c_int32(0)
+            sm = (C.c_double * 16)()
+            rc = self._lib.gf_gmg_cycle_trace(self._h, _lib.ptr(r), _lib.ptr(out),
+                                              0 if cycle == "v" else 1, visits, cap,
+                                              C.byref(nv), sm, _lib.stream_handle())
+            if rc != 0 and nv.value > cap:
+                cap = nv.value
+                continue
+            _lib.check(rc, "gf_gmg_cycle_trace")
+            break
+        self._last_trace_output = out
+        sm = list(sm)
+        chol = self.coarse_chol_mode
+        enrich_note = ("root_enrichment:" + self._root_enrichment_mode
+                       if self._root_enrichment_mode != "none" else "")
+        out_visits = []
+        for k in range(nv.value):
+            t = visits[k]
+            v = list(t.v)
+            if t.is_coarsest:
+                rn, xn, res = v[0], v[7], v[10]
+                out_visits.append(MultigridCycleVisit(
+                    visit_id=t.visit_id, level=t.level, cycle_type=cycle, is_coarsest=True,
+                    n_coarse_corrections=0, rhs_norm=rn, jacobi_action_norm=v[1],
+                    presmooth_solution_norm=0.0, presmooth_residual_norm=rn,
+                    presmooth_residual_ratio=1.0, restricted_rhs_norm=0.0,
+                    restricted_rhs_ratio=0.0, projector_norm=0.0, coarse_space_gap_norm=0.0,
+                    coarse_space_gap_ratio=0.0, coarse_correction_norm=xn,
+                    prolongated_correction_norm=xn, correction_alignment=0.0,
+                    correction_scale_raw=None, correction_scale_applied=1.0,
+                    corrected_residual_norm=res,
+                    corrected_residual_vs_presmooth=_safe_ratio(res, rn),
+                    postsmooth_solution_norm=xn, postsmooth_residual_norm=res,
+                    postsmooth_residual_ratio=_safe_ratio(res, rn), coarsest_mode=chol))
+                continue
+            rn, prer, pec, cr1 = v[0], v[3], v[8], v[10]
+            raw1, app1, note1 = self._coarse_decision(v[11:18])
+            kw = {}
+            notes2 = ""
+            corrected = cr1
+            if t.n_coarse_corrections == 2:
+                raw2, app2, note2 = self._coarse_decision(v[23:30])
+                cr2 = v[22]
+                kw = dict(second_restricted_rhs_norm=v[18], second_coarse_correction_norm=v[19],
+                          second_prolongated_correction_norm=v[20],
+                          second_correction_alignment=_alignment(cr1, v[20], v[21]),
+                          second_correction_scale_raw=raw2, second_correction_scale_applied=app2,
+                          second_corrected_residual_norm=cr2,
+                          second_corrected_residual_vs_corrected=_safe_ratio(cr2, cr1))
+                notes2 = note2
+                if cycle == "v":
+                    corrected = cr2
+            out_visits.append(MultigridCycleVisit(
+                visit_id=t.visit_id, level=t.level, cycle_type=cycle, is_coarsest=False,
+                n_coarse_corrections=int(t.n_coarse_corrections), rhs_norm=rn,
+                jacobi_action_norm=v[1], presmooth_solution_norm=v[2],
+                presmooth_residual_norm=prer, presmooth_residual_ratio=_safe_ratio(prer, rn),
+                restricted_rhs_norm=v[4], restricted_rhs_ratio=_safe_ratio(v[4], prer),
+                projector_norm=v[5], coarse_space_gap_norm=v[6],
+                coarse_space_gap_ratio=_safe_ratio(v[6], prer), coarse_correction_norm=v[7],
+                prolongated_correction_norm=pec, correction_alignment=_alignment(prer, pec, v[9]),
+                correction_scale_raw=raw1, correction_scale_applied=app1,
+                corrected_residual_norm=corrected,
+                corrected_residual_vs_presmooth=_safe_ratio(corrected, prer),
+                postsmooth_solution_norm=v[30], postsmooth_residual_norm=v[31],
+                postsmooth_residual_ratio=_safe_ratio(v[31], rn),
+                notes=_merge_notes(note1, notes2, enrich_note if t.enrichment else "",
+                                   "extra_fine_postsmooth" if t.extra_fine_postsmooth else ""),
+                **kw))
+        raw, applied, lnote, res_norm = self._root_local_decision(sm)
+        if lnote:
+            root = next((x for x in out_visits if x.level == 0 and not x.is_coarsest), None)
+            if root is not None:
+                root.notes = _merge_notes(root.notes, lnote)
+        rhs_n, jac_n, out_n, rz, pre = sm[0], sm[1], sm[2], sm[3], sm[4]
+        ap = self.adaptive_params
+        return MultigridCycleTrace(
+            cycle_type=cycle, rhs_norm=rhs_n, jacobi_action_norm=jac_n, output_norm=out_n,
+            z_over_jacobi=_safe_ratio(out_n, jac_n), rz=rz, pd=bool(rz > 0.0),
+            output_residual_norm=res_norm, output_residual_ratio=_safe_ratio(res_norm, rhs_n),
+            contrast_ratio=self.contrast_ratio, is_high_contrast=self.is_high_contrast,
+            fine_degree_adapt=ap["fine_degree_adapt"], coarse_iters_adapt=ap["coarse_iters_adapt"],
+            coarse_correction_policy=self._coarse_correction_policy,
+            coarse_correction_acceptance_ratio=ap["coarse_correction_acceptance_ratio"],
+            coarse_correction_max_scale=ap["coarse_correction_max_scale"],
+            root_enrichment_mode=self._root_enrichment_mode,
+            root_enrichment_weight_floor=ap["root_enrichment_weight_floor"],
+            root_enrichment_weight_power=ap["root_enrichment_weight_power"],
+            root_local_correction_mode=self._root_local_correction_mode,
+            root_local_correction_inner_steps=ap["root_local_correction_inner_steps"],
+            root_local_correction_inner_restart=ap["root_local_correction_inner_restart"],
+            root_local_correction_raw_scale=raw, root_local_correction_applied_scale=applied,
+            root_local_correction_residual_ratio=_safe_ratio(res_norm, pre),
+            inner_krylov_steps=ap["inner_krylov_steps"],
+            inner_krylov_restart=ap["inner_krylov_restart"], visits=out_visits)
 
     # ------------------------------------------------ level read-outs (tests)
     def level_matvec(self, level: int, x):
