@@ -31,12 +31,12 @@ for p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per FP64 matvec launch on the 80x40x20 grid (LDS-tiled kernel),
-# from rocprofv3 --pmc FETCH_SIZE (1297.4 KiB) and WRITE_SIZE (1615.0 KiB) per
+# from rocprofv3 --pmc FETCH_SIZE (1302.9 KiB) and WRITE_SIZE (1615.0 KiB) per
 # dispatch, each scaled by what the counter reads for a 64 MiB device copy
-# (FETCH 32779.5 KiB -> x1.99927; WRITE 65536 KiB -> x1.0): 2.66 MB read +
+# (FETCH 32783.4 KiB -> x1.99906; WRITE 65536 KiB -> x1.0): 2.67 MB read +
 # 1.65 MB written against 2.17 + 1.65 MB algorithmic
 # (profiles/fine_matvec_fp64_80x40x20_roofline.md)
-TRAFFIC_BYTES = 4309925
+TRAFFIC_BYTES = 4320893
 SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
                  8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
 

@@ -108,27 +108,6 @@ int level1_assemble(const Grid& gc, const DofMap& mapc, const Grid& gf, const do
 // ---------------------------------------------------------------------------
 // block-stencil operator with the shared epilogues
 // ---------------------------------------------------------------------------
-template <typename T, int EPI>
-__device__ __forceinline__ void coarse_epi(int i, T ax, const T* __restrict__ x,
-                                           T* __restrict__ y, const EpiArgs<T>& ep) {
-  if (EPI == EPI_PLAIN) {
-    y[i] = ax;
-  } else if (EPI == EPI_RESID) {
-    y[i] = ep.b[i] - ax;
-  } else if (EPI == EPI_CHEB0) {
-    T z = ep.dinv[i] * (ep.b[i] - ax);
-    ep.dv[i] = z;
-    ep.xo[i] = x[i] + ep.a * z;
-  } else if (EPI == EPI_CHEB) {
-    T z = ep.dinv[i] * (ep.b[i] - ax);
-    T d = ep.a * (z + ep.c * ep.dv[i]);
-    ep.dv[i] = d;
-    ep.xo[i] = x[i] + d;
-  } else {
-    ep.xo[i] = x[i] + (ep.a * ep.dinv[i]) * (ep.b[i] - ax);
-  }
-}
-
 // One thread per (node, stencil slot): a block covers 8 nodes x 32 lanes
 // (27 slots used); slot products Sum_q A[k][r][q] x[J_k, q] meet in LDS and
 // lane r < 3 of each node adds them in slot order before the epilogue.  The
@@ -170,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
   T acc = 0;
 #pragma unroll
   for (int kk = 0; kk < 27; ++kk) acc += part[ln][kk][k];
-  coarse_epi<T, EPI>(o, acc, x, y, ep);
+  epilogue1<T, EPI>(o, acc, x, y, ep);
 }
 
 // Node-per-lane form of k_stencil with coalesced coefficient reads: a block
@@ -222,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil_nodes(Grid g, DofMap map,
     T acc = 0;
 #pragma unroll
     for (int kk = 0; kk < 27; ++kk) acc += tk[kk][r][lane];
-    coarse_epi<T, EPI>(o, acc, x, y, ep);
+    epilogue1<T, EPI>(o, acc, x, y, ep);
   }
 }
 

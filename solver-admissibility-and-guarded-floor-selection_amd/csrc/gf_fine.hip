@@ -26,27 +26,6 @@ struct Acc<float> {
   using type = double;
 };
 
-template <typename T, int EPI>
-__device__ __forceinline__ void apply_epilogue(int i, T ax, const T* __restrict__ x,
-                                               T* __restrict__ y, const EpiArgs<T>& ep) {
-  if (EPI == EPI_PLAIN) {
-    y[i] = ax;
-  } else if (EPI == EPI_RESID) {
-    y[i] = ep.b[i] - ax;
-  } else if (EPI == EPI_CHEB0) {
-    T z = ep.dinv[i] * (ep.b[i] - ax);
-    ep.dv[i] = z;
-    ep.xo[i] = x[i] + ep.a * z;
-  } else if (EPI == EPI_CHEB) {
-    T z = ep.dinv[i] * (ep.b[i] - ax);
-    T d = ep.a * (z + ep.c * ep.dv[i]);
-    ep.dv[i] = d;
-    ep.xo[i] = x[i] + d;
-  } else {  // JACOBI
-    ep.xo[i] = x[i] + (ep.a * ep.dinv[i]) * (ep.b[i] - ax);
-  }
-}
-
 // Element contributions are summed with one fixed tree for every launch
 // shape:  P_ab = E_ab0*t_ab0 + E_ab1*t_ab1  (element (a,b,c) = (ix-1+a,
 // iy-1+b, iz-1+c)),  Q_a = P_a0 + P_a1,  y = Q_0 + Q_1,  each t the 24-term
@@ -131,9 +110,9 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_node(Grid g, DofMap map,
 #pragma unroll
     for (int r = 0; r < 3; ++r) Q[a][r] = P[0][r] + P[1][r];
   }
-  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)(Q[0][0] + Q[1][0]), u, y, ep);
-  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)(Q[0][1] + Q[1][1]), u, y, ep);
-  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)(Q[0][2] + Q[1][2]), u, y, ep);
+  const int o[3] = {o0, o1, o2};
+  const T ax[3] = {(T)(Q[0][0] + Q[1][0]), (T)(Q[0][1] + Q[1][1]), (T)(Q[0][2] + Q[1][2])};
+  epilogue3<T, EPI>(o, ax, u, y, ep);
 }
 
 // SPLIT = 4: a block is 64 nodes x 4 waves; wave w computes the element pair
@@ -223,9 +202,9 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
 #pragma unroll
   for (int q = 0; q < 3; ++q)
     r[q] = (P[q] + part[0][q][lane]) + (part[1][q][lane] + part[2][q][lane]);
-  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)r[0], u, y, ep);
-  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)r[1], u, y, ep);
-  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
+  const int o[3] = {o0, o1, o2};
+  const T ax[3] = {(T)r[0], (T)r[1], (T)r[2]};
+  epilogue3<T, EPI>(o, ax, u, y, ep);
 }
 
 // LDS-tiled form of the four-wave kernel.  A block covers a TY x TZ patch of
@@ -256,22 +235,42 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, ix = tile / (tiles_z * tiles_y);
   const int y0 = ty * TY, z0 = tz * TZ;
-  // stage the node window: entry ((xx*WY + yy)*WZ + zz)*3 + q is node
-  // (ix-1+xx, y0-1+yy, z0-1+zz), component q
-  for (int t = threadIdx.x; t < NU; t += kBlock) {
+  // Stage the node window, entry ((xx*WY + yy)*WZ + zz)*3 + q = node
+  // (ix-1+xx, y0-1+yy, z0-1+zz) component q, and the element window, entry
+  // (ex*EY + ey)*EZ + ez = element (ix-1+ex, y0-1+ey, z0-1+ez).  Every load
+  // of a thread is issued before the first LDS write (masked loads read a
+  // valid address), so they are in flight together.
+  constexpr int NUI = (NU + kBlock - 1) / kBlock, NEI = (NE + kBlock - 1) / kBlock;
+  T uv[NUI], ev[NEI];
+#pragma unroll
+  for (int k = 0; k < NUI; ++k) {
+    const int t = threadIdx.x + k * kBlock;
     const int q = t % 3, r = t / 3;
     const int zz = r % WZ, r2 = r / WZ, yy = r2 % WY, xx = r2 / WY;
     const int jx = ix - 1 + xx, jy = y0 - 1 + yy, jz = z0 - 1 + zz;
-    const bool in = (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) & (jz < g.NZ);
-    us[t] = map_load<OFF>(map, u, 3 * ((jx * g.NY + jy) * g.NZ + jz) + q, in);
+    const bool in = (t < NU) & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
+                    (jz < g.NZ);
+    uv[k] = map_load<OFF>(map, u, 3 * ((jx * g.NY + jy) * g.NZ + jz) + q, in);
   }
-  // element window: entry (ex*EY + ey)*EZ + ez is element (ix-1+ex, y0-1+ey, z0-1+ez)
-  for (int t = threadIdx.x; t < NE; t += kBlock) {
+#pragma unroll
+  for (int k = 0; k < NEI; ++k) {
+    const int t = threadIdx.x + k * kBlock;
     const int ez = t % EZ, r = t / EZ, ey = r % EY, ex = r / EY;
     const int cx = ix - 1 + ex, cy = y0 - 1 + ey, cz = z0 - 1 + ez;
-    const bool in = (cx >= 0) & (cx < g.nx) & (cy >= 0) & (cy < g.ny) & (cz >= 0) & (cz < g.nz);
-    const T ev = E[in ? (cx * g.ny + cy) * g.nz + cz : 0];
-    es[t] = in ? ev : T(0);
+    const bool in = (t < NE) & (cx >= 0) & (cx < g.nx) & (cy >= 0) & (cy < g.ny) & (cz >= 0) &
+                    (cz < g.nz);
+    const T e = E[in ? (cx * g.ny + cy) * g.nz + cz : 0];
+    ev[k] = in ? e : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < NUI; ++k) {
+    const int t = threadIdx.x + k * kBlock;
+    if (t < NU) us[t] = uv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NEI; ++k) {
+    const int t = threadIdx.x + k * kBlock;
+    if (t < NE) es[t] = ev[k];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -306,16 +305,15 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
   }
   __syncthreads();
   if (wv != 0 || !valid) return;
-  const int n = (ix * g.NY + iy) * g.NZ + iz;
-  const int o0 = map_idx<OFF>(map, 3 * n), o1 = map_idx<OFF>(map, 3 * n + 1),
-            o2 = map_idx<OFF>(map, 3 * n + 2);
   A r[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q)
     r[q] = (P[q] + part[0][q][lane]) + (part[1][q][lane] + part[2][q][lane]);
-  if (o0 >= 0) apply_epilogue<T, EPI>(o0, (T)r[0], u, y, ep);
-  if (o1 >= 0) apply_epilogue<T, EPI>(o1, (T)r[1], u, y, ep);
-  if (o2 >= 0) apply_epilogue<T, EPI>(o2, (T)r[2], u, y, ep);
+  const int n = (ix * g.NY + iy) * g.NZ + iz;
+  const int o[3] = {map_idx<OFF>(map, 3 * n), map_idx<OFF>(map, 3 * n + 1),
+                    map_idx<OFF>(map, 3 * n + 2)};
+  const T ax[3] = {(T)r[0], (T)r[1], (T)r[2]};
+  epilogue3<T, EPI>(o, ax, u, y, ep);
 }
 
 // Element strain energy and SIMP sensitivity (solver_v4.py:258-265):

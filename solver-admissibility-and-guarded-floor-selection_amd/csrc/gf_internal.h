@@ -45,6 +45,85 @@ struct EpiArgs {
   T c = 0;
 };
 
+// Explicitly rounded arithmetic for the operator epilogues: every product
+// and sum is one IEEE operation (no contraction left to the compiler), so
+// every kernel that applies an epilogue produces the same bits whatever code
+// surrounds it.
+__device__ __forceinline__ double rn_mul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float rn_mul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double rn_add(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ float rn_add(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double rn_sub(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ float rn_sub(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ double rn_fma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float rn_fma(float a, float b, float c) { return fmaf(a, b, c); }
+
+// The operator epilogue at one output index i from its loaded operands
+// (multigrid_v4.py:1783-1812 smoothing steps, solver_v2.py residuals):
+//   PLAIN  y = A x               RESID  y = b - A x
+//   CHEB0  z = D^-1 (b - A x), dv = z, xo = x + a z
+//   CHEB   z = D^-1 (b - A x), d = a (z + c dv), dv = d, xo = x + d
+//   JACOBI xo = x + (a D^-1)(b - A x)
+template <typename T, int EPI>
+__device__ __forceinline__ void epi_store(int i, T ax, T b, T di, T dv, T xv, T* __restrict__ y,
+                                          const EpiArgs<T>& ep) {
+  if (EPI == EPI_PLAIN) {
+    y[i] = ax;
+  } else if (EPI == EPI_RESID) {
+    y[i] = rn_sub(b, ax);
+  } else if (EPI == EPI_CHEB0) {
+    const T z = rn_mul(di, rn_sub(b, ax));
+    ep.dv[i] = z;
+    ep.xo[i] = rn_fma(ep.a, z, xv);
+  } else if (EPI == EPI_CHEB) {
+    const T z = rn_mul(di, rn_sub(b, ax));
+    const T d = rn_mul(ep.a, rn_fma(ep.c, dv, z));
+    ep.dv[i] = d;
+    ep.xo[i] = rn_add(xv, d);
+  } else {
+    ep.xo[i] = rn_fma(rn_mul(ep.a, di), rn_sub(b, ax), xv);
+  }
+}
+
+template <int EPI>
+constexpr bool epi_reads_x() {
+  return EPI == EPI_CHEB0 || EPI == EPI_CHEB || EPI == EPI_JACOBI;
+}
+
+// Epilogue at one output index (i >= 0).
+template <typename T, int EPI>
+__device__ __forceinline__ void epilogue1(int i, T ax, const T* __restrict__ x,
+                                          T* __restrict__ y, const EpiArgs<T>& ep) {
+  const T b = EPI != EPI_PLAIN ? ep.b[i] : T(0);
+  const T di = epi_reads_x<EPI>() ? ep.dinv[i] : T(0);
+  const T xv = epi_reads_x<EPI>() ? x[i] : T(0);
+  const T dv = EPI == EPI_CHEB ? ep.dv[i] : T(0);
+  epi_store<T, EPI>(i, ax, b, di, dv, xv, y, ep);
+}
+
+// Epilogue of the three DOFs of one node (o[q] < 0: fixed DOF, skipped).
+// Every operand is loaded before the first store: a store through one
+// pointer may alias a later load through another as far as the compiler
+// knows, which otherwise serialises the three DOFs' loads.  The indices of
+// one node are distinct, so the early loads read what epilogue1 would.
+template <typename T, int EPI>
+__device__ __forceinline__ void epilogue3(const int (&o)[3], const T (&ax)[3],
+                                          const T* __restrict__ x, T* __restrict__ y,
+                                          const EpiArgs<T>& ep) {
+  T b[3], di[3], dv[3], xv[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int i = o[q] >= 0 ? o[q] : 0;
+    b[q] = EPI != EPI_PLAIN ? ep.b[i] : T(0);
+    di[q] = epi_reads_x<EPI>() ? ep.dinv[i] : T(0);
+    xv[q] = epi_reads_x<EPI>() ? x[i] : T(0);
+    dv[q] = EPI == EPI_CHEB ? ep.dv[i] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (o[q] >= 0) epi_store<T, EPI>(o[q], ax[q], b[q], di[q], dv[q], xv[q], y, ep);
+}
+
 // ---- BLAS-1 (gf_blas.hip) --------------------------------------------------
 int dot_f64(Work& w, long n, const double* x, const double* y, double* out, bool sqrt_,
             const int* gate, hipStream_t st);

@@ -366,6 +366,31 @@ def test_fine_launch_shapes_bitwise_identical(dims, kind):
     assert shape in shapes
 
 
+@pytest.mark.parametrize("rho_min", [1e-12, 1e-3])
+def test_preconditioner_bitwise_across_launch_shapes(rho_min):
+    """The whole preconditioner (every smoother epilogue: Chebyshev start and
+    step, residuals) gives the same bits with every fine launch shape, so the
+    per-context autotuner cannot change results.  Eager launches (no graph),
+    one hierarchy re-set-up per shape."""
+    from gpu_fem import _lib, policy as gpol
+    from gpu_fem.presets import cantilever
+    from gpu_fem.solver_v2 import LAUNCH_SHAPES
+
+    s = _setup((16, 8, 8), "tip", rho_min=rho_min)
+    comps = gpol._build_components(cantilever(16, 8, 8), **dict(STACK, graph_apply=False))
+    mf, gg = comps[6], comps[7]
+    r = _t(np.random.default_rng(5).standard_normal(len(s["free"])))
+    outs = {}
+    for _, shape in LAUNCH_SHAPES:
+        _lib.call("gf_ctx_set_shape", mf.ctx_handle, shape)
+        gg.setup(s["Et"])
+        outs[shape] = (_n(gg.apply(r)), _n(gg.vcycle(r)))
+    first = LAUNCH_SHAPES[0][1]
+    for _, shape in LAUNCH_SHAPES[1:]:
+        for k in range(2):
+            assert np.array_equal(outs[first][k], outs[shape][k]), (shape, k)
+
+
 def test_optimized_density_c64_golden(golden_dir):
     """Reference anchor: the shipped C64_MF optimized density (80x40x20
     cantilever, penal 4.5, tip load) at rho_min=1e-12 with the canonical stack
