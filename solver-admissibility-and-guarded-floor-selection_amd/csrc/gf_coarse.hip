@@ -175,10 +175,16 @@ __global__ __launch_bounds__(kBlock) void k_stencil_nodes(Grid g, DofMap map,
   int ix, iy, iz;
   node_xyz(g, nn_, ix, iy, iz);
   const long nn = g.n_nodes;
-  for (int k = wv; k < 27; k += 4) {
+  // wave wv owns slots wv, wv+4, ..., wv+24 (the last one for wv < 3 only,
+  // masked rather than branched over): a compile-time trip count with no
+  // branch between the rounds lets the wave's loads be issued together
+#pragma unroll
+  for (int t = 0; t < 7; ++t) {
+    const bool kin = (t < 6) | (wv < 3);
+    const int k = kin ? wv + 4 * t : 0;
     const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
-    const bool in = valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
-                    (jz < g.NZ);
+    const bool in = kin & valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) &
+                    (jz >= 0) & (jz < g.NZ);
     const int m = (jx * g.NY + jy) * g.NZ + jz;
     T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
@@ -188,9 +194,11 @@ __global__ __launch_bounds__(kBlock) void k_stencil_nodes(Grid g, DofMap map,
       s1 += A[(k * 9 + 1 * 3 + q) * nn + nn_] * xv;
       s2 += A[(k * 9 + 2 * 3 + q) * nn + nn_] * xv;
     }
-    tk[k][0][lane] = s0;
-    tk[k][1][lane] = s1;
-    tk[k][2][lane] = s2;
+    if (kin) {
+      tk[k][0][lane] = s0;
+      tk[k][1][lane] = s1;
+      tk[k][2][lane] = s2;
+    }
   }
   __syncthreads();
   if (wv != 0 || !valid) return;
