@@ -335,22 +335,30 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid g
   if ((o0 & o1 & o2) < 0) return;
   int ix, iy, iz;
   node_xyz(gf, n, ix, iy, iz);
-  int px[2], py[2], pz[2];
-  double wx[2], wy[2], wz[2], sx, sy, sz;
+  int px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
+  double wx[2] = {0.0, 0.0}, wy[2] = {0.0, 0.0}, wz[2] = {0.0, 0.0}, sx, sy, sz;
   const int cx = parents_1d(ix, gc.NX, px, wx, sx);
   const int cy = parents_1d(iy, gc.NY, py, wy, sy);
   const int cz = parents_1d(iz, gc.NZ, pz, wz, sz);
   const double inv = 1.0 / ((sx * sy) * sz);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int a = 0; a < cx; ++a)
-    for (int b = 0; b < cy; ++b)
-      for (int c = 0; c < cz; ++c) {
+  // all 2x2x2 parent slots with a fixed trip count (slots past a parent
+  // count are masked), so the up to 24 loads issue together; the terms are
+  // added in the same (a, b, c) order as before
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const bool in = (a < cx) & (b < cy) & (c < cz);
         const double w = ((wx[a] * wy[b]) * wz[c]) * inv;
         const int m = (px[a] * gc.NY + py[b]) * gc.NZ + pz[c];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          const int f = mapc(3 * m + q);
-          if (f >= 0) acc[q] += w * ec[f];
+          const int f = in ? mapc(3 * m + q) : -1;
+          const double v = ec[f >= 0 ? f : 0];
+          if (f >= 0) acc[q] += w * v;
         }
       }
   if (o0 >= 0) yf[o0] = acc[0];
@@ -385,15 +393,22 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
       wy[a] = (fy >= 0 && fy < gf.NY) ? pweight(fy, iy, gc.NY) : 0.0;
       wz[a] = (fz >= 0 && fz < gf.NZ) ? pweight(fz, iz, gc.NZ) : 0.0;
     }
-    for (int c27 = grp; c27 < 27; c27 += 4) {
+    // the group's children c27 = grp, grp+4, ..., grp+24 (the last for
+    // grp < 3 only) with a fixed trip count, absent children masked, so the
+    // loads issue together; terms are added in the same order as before
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      const bool kin = (t < 6) | (grp < 3);
+      const int c27 = kin ? grp + 4 * t : 0;
       const int a = c27 / 9, b = (c27 / 3) % 3, c = c27 % 3;
-      if (wx[a] == 0.0 || wy[b] == 0.0 || wz[c] == 0.0) continue;
+      const bool in = kin & (wx[a] != 0.0) & (wy[b] != 0.0) & (wz[c] != 0.0);
       const double w = (wx[a] * wy[b]) * wz[c];
       const int m = ((2 * ix + a - 1) * gf.NY + (2 * iy + b - 1)) * gf.NZ + (2 * iz + c - 1);
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const int f = mapf(3 * m + q);
-        if (f >= 0) acc[q] += w * rf[f];
+        const int f = in ? mapf(3 * m + q) : -1;
+        const double v = rf[f >= 0 ? f : 0];
+        if (f >= 0) acc[q] += w * v;
       }
     }
   }

@@ -293,11 +293,30 @@ POLICY_DEFAULTS = dict(penal=3.0, raised_rho_mins=FLOORS[1:], baseline_rho_min=F
 
 def _run_chunked(session, sts, chunk, progress, tag):
     """session.run over consecutive chunks (one session, so the cached
-    spectral estimate of its first state carries on), reporting after each."""
+    lambda and the concurrency carry over).  ``progress`` gets a line after
+    every chunk, and after any state that finishes 30 s or more after the
+    previous line (long states on large grids)."""
+    import threading
+    import time
+
     out = []
+    lock = threading.Lock()
+    last = [time.perf_counter()]
+
+    def on_done(_i):
+        if progress is None:
+            return
+        with lock:
+            now = time.perf_counter()
+            if now - last[0] >= 30.0:
+                last[0] = now
+                progress(f"{tag}: a state finished ({len(out)}/{len(sts)} before this chunk)")
+
     for k in range(0, len(sts), chunk):
-        out.extend(session.run(sts[k:k + chunk]))
+        out.extend(session.run(sts[k:k + chunk], on_done=on_done))
         if progress is not None:
+            with lock:
+                last[0] = time.perf_counter()
             progress(f"{tag}: {len(out)}/{len(sts)} states")
     return out
 

@@ -340,10 +340,12 @@ class PolicySession:
             _sync()
         return row
 
-    def run(self, states):
+    def run(self, states, on_done=None):
         """states: list of dicts with ``solid`` (or ``rho_field``) and optional
         ``seed`` / ``probability`` / ``keep_x`` / ``extra`` (keyword arguments
-        passed on to the runner's ``run_state``).  Returns the rows in order."""
+        passed on to the runner's ``run_state``).  Returns the rows in order.
+        ``on_done(i)`` (optional) is called as each state finishes, from the
+        thread that ran it."""
         import queue
         import threading
 
@@ -354,6 +356,8 @@ class PolicySession:
         if not self._started and states:
             st0 = states[0]
             rows[0] = self._run_one(0, st0)
+            if on_done is not None:
+                on_done(0)
             E0 = torch.as_tensor(self.runners[0].first_moduli(st0), device="cuda")
             for k in range(1, self.workers):
                 with torch.cuda.stream(self.streams[k]):
@@ -374,6 +378,8 @@ class PolicySession:
                     return
                 try:
                     rows[i] = self._run_one(k, states[i])
+                    if on_done is not None:
+                        on_done(i)
                 except BaseException as ex:  # surfaced in the caller
                     errors.append(ex)
                     return
