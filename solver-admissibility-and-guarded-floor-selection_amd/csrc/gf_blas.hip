@@ -30,9 +30,21 @@ __global__ __launch_bounds__(kBlock) void k_dot(long n, const T* __restrict__ x,
                                                 const int* gate) {
   if (gate && *gate == 0) return;
   double a[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
-    a[0] += (double)x[i] * (double)y[i];
+  // four strides' loads issue before the first is used; the sum is taken
+  // in the same order as the plain stride loop (identical bits)
+  const long st = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    T xv[4], yv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xv[u] = x[i + u * st];
+      yv[u] = y[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[0] += (double)xv[u] * (double)yv[u];
+  }
+  for (; i < n; i += st) a[0] += (double)x[i] * (double)y[i];
   finish_reduce<1>(a, partials, ticket, out, post);
 }
 
@@ -47,8 +59,26 @@ __global__ __launch_bounds__(kBlock) void k_dot4_ls(long n, const double* __rest
                                                     const int* gate) {
   if (gate && *gate == 0) return;
   double a[4] = {0.0, 0.0, 0.0, 0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
+  // four strides' loads in flight, sums in the plain loop's order
+  const long st = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    double cv[4], rv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cv[u] = c[i + u * st];
+      rv[u] = r[i + u * st];
+      av[u] = Ac[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[0] += cv[u] * cv[u];
+      a[1] += rv[u] * rv[u];
+      a[2] += av[u] * av[u];
+      a[3] += rv[u] * av[u];
+    }
+  }
+  for (; i < n; i += st) {
     double cv = c[i], rv = r[i], av = Ac[i];
     a[0] += cv * cv;
     a[1] += rv * rv;
@@ -69,8 +99,25 @@ __global__ __launch_bounds__(kBlock) void k_mgs_axpy_dot(long n, const double* _
   if (gate && *gate == 0) return;
   const double h = *hptr;
   double a[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
+  // four strides' loads in flight, sums in the plain loop's order
+  const long st = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    double vv[4], ww[4], nx[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      vv[u] = v[i + u * st];
+      ww[u] = w[i + u * st];
+      nx[u] = next ? next[i + u * st] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double wv = ww[u] - h * vv[u];
+      w[i + u * st] = wv;
+      a[0] += (next ? nx[u] : wv) * wv;
+    }
+  }
+  for (; i < n; i += st) {
     double wv = w[i] - h * v[i];
     w[i] = wv;
     a[0] += (next ? next[i] : wv) * wv;
@@ -277,8 +324,24 @@ __global__ __launch_bounds__(kBlock) void k_trial_norm(long n, const double* __r
   if (gate && *gate == 0) return;
   const double s = *sptr;
   double a[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
+  // four strides' loads in flight, sums in the plain loop's order
+  const long st = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    double bv[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      bv[u] = b[i + u * st];
+      xv[u] = x[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double t = bv[u] - s * xv[u];
+      if (y) y[i + u * st] = t;
+      a[0] += t * t;
+    }
+  }
+  for (; i < n; i += st) {
     double t = b[i] - s * x[i];
     if (y) y[i] = t;
     a[0] += t * t;
