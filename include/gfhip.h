@@ -133,7 +133,8 @@ int32_t gf_ctx_destroy(gf_ctx* ctx);
  * times the fine-operator kernels on this grid -- shape 1 one lane per node,
  * 4 four waves per 64 nodes, 8 and 16 the same with the node and element
  * windows staged in LDS for an 8x8 / 4x16 (y,z) patch -- and keeps the
- * fastest for every later call on ctx.  All shapes give bitwise-identical
+ * fastest for every later call on ctx (each shape's best of three
+ * interleaved timing rounds).  All shapes give bitwise-identical
  * results.  us_out (nullable, host): 4 floats of timings in that order;
  * shape_out (nullable, host): the chosen shape.  Synchronises. */
 int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* stream);

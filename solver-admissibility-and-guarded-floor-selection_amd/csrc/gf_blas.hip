@@ -384,7 +384,16 @@ __global__ __launch_bounds__(kBlock) void k_combine(long n, const double* __rest
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
     double acc = accumulate ? x[i] : 0.0;
-    for (int j = 0; j < m; ++j) acc = acc + ycoef[j] * Z[j * ldz + i];
+    // eight basis vectors' loads in flight, accumulated in j order
+    int j = 0;
+    for (; j + 8 <= m; j += 8) {
+      double z[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) z[u] = Z[(long)(j + u) * ldz + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = acc + ycoef[j + u] * z[u];
+    }
+    for (; j < m; ++j) acc = acc + ycoef[j] * Z[(long)j * ldz + i];
     x[i] = acc;
   }
 }
@@ -530,7 +539,10 @@ int scale_inv(long n, const double* x, const double* h, double thresh, double* y
 
 int combine(long n, const double* Z, long ldz, const double* y, const int* jcount, bool acc,
             double* x, const int* gate, hipStream_t st) {
-  hipLaunchKernelGGL(k_combine, dim3(red_grid(n)), dim3(kBlock), 0, st, n, Z, ldz, y, jcount,
+  // one element per thread up to 4096 blocks (no cross-thread sum: the grid
+  // does not change the result)
+  const int grid = (int)std::min<long>(4096, std::max<long>(1, cdiv(n, kBlock)));
+  hipLaunchKernelGGL(k_combine, dim3(grid), dim3(kBlock), 0, st, n, Z, ldz, y, jcount,
                      acc ? 1 : 0, x, gate);
   GF_CHECK_LAUNCH();
   return 0;

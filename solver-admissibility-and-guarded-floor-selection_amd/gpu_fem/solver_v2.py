@@ -11,6 +11,7 @@ arrays); all arithmetic runs in libgfhip.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 
 import numpy as np
@@ -109,7 +110,15 @@ class MatrixFreeKff:
         self._KE_diag = np.diag(self._ctx.KE).copy()
         self.launch_shape = None
         self.launch_timings_us = None
-        if autotune:
+        forced = int(os.environ.get("GFHIP_FINE_SHAPE", "0") or 0)
+        if forced:
+            # a fixed shape instead of timing (for profiling runs, where the
+            # tracer's per-dispatch cost can reorder the shapes' timings)
+            if forced not in {code for _, code in LAUNCH_SHAPES}:
+                raise ValueError(f"GFHIP_FINE_SHAPE must be one of 1, 4, 8, 16; got {forced}")
+            _lib.call("gf_ctx_set_shape", self._ctx.h, forced)
+            self.launch_shape = forced
+        elif autotune:
             self.autotune()
 
     def autotune(self):
