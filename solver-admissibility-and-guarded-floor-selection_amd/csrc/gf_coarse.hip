@@ -371,7 +371,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid g
 // partial sums combined in LDS in group order.
 __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
                                                      const double* __restrict__ rf,
-                                                     double* __restrict__ yc, const int* gate) {
+                                                     double* __restrict__ yc, ZeroStart zs,
+                                                     const int* gate) {
   if (gate && *gate == 0) return;
   __shared__ double part[3][3][64];
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
@@ -421,7 +422,17 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const double v = ((acc[q] + part[0][q][lane]) + part[1][q][lane]) + part[2][q][lane];
-    if (o[q] >= 0) yc[o[q]] = v;
+    const int i = o[q];
+    if (i >= 0) {
+      yc[i] = v;
+      if (zs.mode == 1) {
+        const double z = rn_mul(zs.dinv[i], v);
+        zs.dv[i] = z;
+        zs.xo[i] = rn_add(0.0, rn_mul(zs.a, z));
+      } else if (zs.mode == 2) {
+        zs.xo[i] = rn_add(0.0, rn_mul(rn_mul(zs.a, zs.dinv[i]), v));
+      }
+    }
   }
 }
 
@@ -434,9 +445,11 @@ int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& ma
 }
 
 int restrict_(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
-              const double* rf, double* yc, const int* gate, hipStream_t st) {
+              const double* rf, double* yc, const int* gate, hipStream_t st,
+              const ZeroStart* zs) {
+  const ZeroStart z = zs ? *zs : ZeroStart{};
   hipLaunchKernelGGL(k_restrict, dim3(cdiv(gc.n_nodes, 64)), dim3(kBlock), 0, st, gf, mapf,
-                     gc, mapc, rf, yc, gate);
+                     gc, mapc, rf, yc, z, gate);
   GF_CHECK_LAUNCH();
   return 0;
 }

@@ -439,12 +439,13 @@ __global__ __launch_bounds__(kBlock) void k_zero_start(long n, const T* __restri
   if (gate && *gate == 0) return;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
+    // explicitly rounded, as in the restriction's fused form (ZeroStart)
     if (dv) {
-      T z = dinv[i] * b[i];
+      const T z = rn_mul(dinv[i], b[i]);
       dv[i] = z;
-      x[i] = T(0) + a * z;
+      x[i] = rn_add(T(0), rn_mul(a, z));
     } else {
-      x[i] = T(0) + (a * dinv[i]) * b[i];
+      x[i] = rn_add(T(0), rn_mul(rn_mul(a, dinv[i]), b[i]));
     }
   }
 }

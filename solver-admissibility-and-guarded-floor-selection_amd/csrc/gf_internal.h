@@ -261,8 +261,21 @@ void set_stencil_form(int form);
 int stencil_diag(const Grid& g, const DofMap& map, const double* A, double* diag, hipStream_t st);
 int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
             const double* ec, double* yf, const int* gate, hipStream_t st);
+// First smoothing step from a zero start on the coarse level, applied by the
+// restriction that produces that level's right-hand side b (one launch
+// fewer per coarse level visit): mode 1 Chebyshev, z = D^-1 b, dv = z,
+// xo = 0 + a z; mode 2 Jacobi, xo = 0 + (a D^-1) b.  The same expressions as
+// zero_start_step, so the bits are the same.
+struct ZeroStart {
+  int mode = 0;
+  const double* dinv = nullptr;
+  double* dv = nullptr;
+  double* xo = nullptr;
+  double a = 0.0;
+};
 int restrict_(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
-              const double* rf, double* yc, const int* gate, hipStream_t st);
+              const double* rf, double* yc, const int* gate, hipStream_t st,
+              const ZeroStart* zs = nullptr);
 int rap(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
         const double* Af, double* KP, double* Ac, hipStream_t st);
 int stencil_to_dense(const Grid& g, const DofMap& map, const double* A, int n_free, double* D,

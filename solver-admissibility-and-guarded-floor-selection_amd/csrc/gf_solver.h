@@ -175,7 +175,11 @@ struct Gmg {
               const int* gate, hipStream_t st);
   int solve_coarsest(const double* rhs, double* out, const int* gate, hipStream_t st,
                      double** result);
-  int enrichment_rhs(const double* residual, double* out, const int* gate, hipStream_t st);
+  int enrichment_rhs(const double* residual, double* out, const int* gate, hipStream_t st,
+                     const ZeroStart* zs = nullptr);
+  const ZeroStart* fuse_zero_start(int lc);
+  ZeroStart zs_{};
+  int zs_level = -1;  // level whose next zero-start step a restriction applied
   int fgmres_dev(Krylov& K, const std::function<int(const double*, double*, const int*)>& M,
                  const double* b, double* x, double tol, int maxiter, int restart,
                  hipStream_t st);

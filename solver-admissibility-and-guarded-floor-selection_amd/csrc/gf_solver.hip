@@ -738,6 +738,44 @@ __global__ void k_cast_gated_d2f(long n, const double* a, float* b, const int* g
     b[i] = (float)a[i];
 }
 
+// Chebyshev coefficients of level l in precision T (multigrid_v4.py:1783-1812):
+// returns the first step's alpha and sets sigma, delta.
+template <typename T>
+static T cheb_coeffs(const Gmg& G, int l, T& sigma, T& delta) {
+  double lamd = l == 0 ? G.lam0 : G.lam[l];
+  if (l > 0 && !std::isfinite(lamd)) lamd = 1.0;
+  double lfd = G.p.cheb_lower_frac;
+  if (G.high) lfd = std::min(G.p.cheb_lower_frac, 1.0 / 100.0);
+  const T lam = (T)lamd, lf = (T)lfd;
+  const T lam_min = lf * lam;
+  sigma = (lam + lam_min) * (T)0.5;
+  delta = (lam - lam_min) * (T)0.5;
+  return (T)2.0 / (lam + lam_min);
+}
+
+// The zero-start first smoothing step of coarse level lc, for the
+// restriction that produces its right-hand side to apply (FP64 levels that
+// are not the coarsest; the pre-smoother from zero is the first thing the
+// level's cycle does).  Marks the level so the smoother skips that step.
+const ZeroStart* Gmg::fuse_zero_start(int lc) {
+  if (lc <= 0 || lc >= L - 1 || lv[lc].prec != 0) return nullptr;
+  Level& v = lv[lc];
+  zs_ = ZeroStart{};
+  zs_.dinv = v.dinv;
+  zs_.xo = v.X[0];
+  if (p.smoother_type == 0) {
+    double sigma, delta;
+    zs_.mode = 1;
+    zs_.dv = v.DV;
+    zs_.a = cheb_coeffs<double>(*this, lc, sigma, delta);
+  } else {
+    zs_.mode = 2;
+    zs_.a = omega_used;
+  }
+  zs_level = lc;
+  return &zs_;
+}
+
 // Generic smoother body in precision T.  xs[2] are the ping-pong buffers,
 // x_in (nullable = zero start) must not alias them unless it is xs[cur].
 template <typename T>
@@ -755,19 +793,16 @@ static int smooth_T(Gmg& G, int l, const T* x_in, const T* rhs, const T* dinv, T
     nb ^= 1;
     return d;
   };
+  // a zero-start first step already applied by the restriction that
+  // produced rhs (Gmg::fuse_zero_start) is not applied again
+  const bool started = !x && G.zs_level == l;
+  if (started) G.zs_level = -1;
   if (G.p.smoother_type == 0) {
-    double lamd = l == 0 ? G.lam0 : G.lam[l];
-    if (l > 0 && !std::isfinite(lamd)) lamd = 1.0;
-    double lfd = G.p.cheb_lower_frac;
-    if (G.high) lfd = std::min(G.p.cheb_lower_frac, 1.0 / 100.0);
-    const T lam = (T)lamd, lf = (T)lfd;
-    const T lam_min = lf * lam;
-    const T sigma = (lam + lam_min) * (T)0.5;
-    const T delta = (lam - lam_min) * (T)0.5;
-    T alpha = (T)2.0 / (lam + lam_min);
+    T sigma, delta;
+    T alpha = cheb_coeffs<T>(G, l, sigma, delta);
     T* xo = dest(0);
     if (!x) {
-      GF_TRY(zero_start_step<T>(n, rhs, dinv, alpha, dv, xo, gate, st));
+      if (!started) GF_TRY(zero_start_step<T>(n, rhs, dinv, alpha, dv, xo, gate, st));
     } else {
       EpiArgs<T> ep;
       ep.b = rhs;
@@ -798,7 +833,7 @@ static int smooth_T(Gmg& G, int l, const T* x_in, const T* rhs, const T* dinv, T
     for (int k = 0; k < deg; ++k) {
       T* xn = dest(k);
       if (!x) {
-        GF_TRY(zero_start_step<T>(n, rhs, dinv, om, (T*)nullptr, xn, gate, st));
+        if (!started) GF_TRY(zero_start_step<T>(n, rhs, dinv, om, (T*)nullptr, xn, gate, st));
       } else {
         EpiArgs<T> ep;
         ep.b = rhs;
@@ -889,7 +924,8 @@ int Gmg::correct(int l, double* x, const double* residual, const double* corr,
   return 0;
 }
 
-int Gmg::enrichment_rhs(const double* residual, double* out, const int* gate, hipStream_t st) {
+int Gmg::enrichment_rhs(const double* residual, double* out, const int* gate, hipStream_t st,
+                        const ZeroStart* zs) {
   Level& f = lv[0];
   Level& c = lv[1];
   const long n = f.n_free;
@@ -902,7 +938,7 @@ int Gmg::enrichment_rhs(const double* residual, double* out, const int* gate, hi
   }
   LAUNCH(k_enrich, n, st, n, re_w, vm ? 1 : 0, p.root_enrichment_weight_floor, residual,
          gap ? f.T : nullptr, f.AC, gate);
-  return restrict_(f.g, f.map(), c.g, c.map(), f.AC, out, gate, st);
+  return restrict_(f.g, f.map(), c.g, c.map(), f.AC, out, gate, st, zs);
 }
 
 int Gmg::cycle(int l, const double* rhs, double* out, const int* gate, hipStream_t st,
@@ -920,14 +956,14 @@ int Gmg::cycle(int l, const double* rhs, double* out, const int* gate, hipStream
     ep.b = rhs;
     GF_TRY(level_op(l, x, v.R, EPI_RESID, ep, gate, st));
   }
-  GF_TRY(restrict_(v.g, v.map(), c.g, c.map(), v.R, c.RHS, gate, st));
+  GF_TRY(restrict_(v.g, v.map(), c.g, c.map(), v.R, c.RHS, gate, st, fuse_zero_start(l + 1)));
   double* ec = nullptr;
   GF_TRY(cycle(l + 1, c.RHS, nullptr, gate, st, &ec));
   GF_TRY(prolong(v.g, v.map(), c.g, c.map(), ec, v.PE, gate, st));
   if (!wcyc) {
     GF_TRY(correct(l, x, v.R, v.PE, enrich ? v.R2 : nullptr, gate, st));
     if (enrich) {
-      GF_TRY(enrichment_rhs(v.R2, c.RHS, gate, st));
+      GF_TRY(enrichment_rhs(v.R2, c.RHS, gate, st, fuse_zero_start(l + 1)));
       double* e2 = nullptr;
       GF_TRY(cycle(l + 1, c.RHS, nullptr, gate, st, &e2));
       GF_TRY(prolong(v.g, v.map(), c.g, c.map(), e2, v.PE, gate, st));
@@ -943,9 +979,10 @@ int Gmg::cycle(int l, const double* rhs, double* out, const int* gate, hipStream
       GF_TRY(correct(l, x, v.R, v.PE, v.R2, gate, st));
     }
     if (enrich) {
-      GF_TRY(enrichment_rhs(v.R2, c.RHS, gate, st));
+      GF_TRY(enrichment_rhs(v.R2, c.RHS, gate, st, fuse_zero_start(l + 1)));
     } else {
-      GF_TRY(restrict_(v.g, v.map(), c.g, c.map(), v.R2, c.RHS, gate, st));
+      GF_TRY(restrict_(v.g, v.map(), c.g, c.map(), v.R2, c.RHS, gate, st,
+                       fuse_zero_start(l + 1)));
     }
     double* e2 = nullptr;
     GF_TRY(cycle(l + 1, c.RHS, nullptr, gate, st, &e2));
