@@ -222,3 +222,46 @@ def test_confidence_for_accuracy():
     assert confidence_for_accuracy(conf, ok, 0.8) == 0.7
     assert confidence_for_accuracy(conf, ok, 0.7) == 0.5
     assert confidence_for_accuracy(conf, np.zeros(7, bool), 0.5) == 1.0
+
+
+def test_device_lanczos_recurrence_on_the_host():
+    """The package's Lanczos recurrence (``lanczos_kappa``, run on device
+    vectors through the library's dot / nrm2 / axpby in the GPU path) with
+    NumPy stand-ins for those callables: exact max/min on a diagonal operator
+    with n <= steps (breakdown path), and the oracle's recurrence to rounding
+    on a longer spectrum."""
+    from gpu_fem.multigrid_v4 import lanczos_kappa
+    from oracle.gmg import lanczos_kappa as oracle_lanczos
+
+    def axpby(a, x, b, y):
+        y *= b
+        y += a * x
+        return y
+
+    kw = dict(dot_fn=lambda a, b: float(a @ b), nrm_fn=lambda a: float(np.linalg.norm(a)),
+              axpby_fn=axpby, zeros_like=np.zeros_like)
+    lam = np.array([0.5, 1.0, 2.0, 3.0, 7.5, 10.0])
+    v = np.random.default_rng(1).standard_normal(len(lam))
+    v /= np.linalg.norm(v)
+    assert abs(lanczos_kappa(lambda u: lam * u, v.copy(), 30, **kw) - 20.0) <= 1e-10 * 20.0
+    lam = np.linspace(1.0, 100.0, 200)
+    v = np.random.default_rng(2).standard_normal(len(lam))
+    v /= np.linalg.norm(v)
+    for m in (5, 12, 25):
+        a = lanczos_kappa(lambda u: lam * u, v.copy(), m, **kw)
+        b = oracle_lanczos(lambda u: lam * u, v.copy(), m)
+        assert abs(a - b) <= 1e-9 * b, m
+
+
+def test_trace_ratio_rules():
+    """_safe_ratio, _merge_notes and _alignment follow multigrid_v4.py:840-862
+    and 934-936 (zero denominators, empty notes)."""
+    from gpu_fem.multigrid_v4 import _alignment, _merge_notes, _safe_ratio
+
+    assert _safe_ratio(1.0, 2.0) == 0.5
+    assert _safe_ratio(0.0, 0.0) == 0.0
+    assert _safe_ratio(1.0, 0.0) == float("inf")
+    assert _merge_notes("", "a", "", "b") == "a|b"
+    assert _merge_notes("", "") == ""
+    assert _alignment(0.0, 1.0, 5.0) == 0.0
+    assert _alignment(2.0, 3.0, 6.0) == 1.0
