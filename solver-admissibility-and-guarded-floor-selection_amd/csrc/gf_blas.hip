@@ -129,13 +129,18 @@ __global__ __launch_bounds__(kBlock) void k_mgs_axpy_dot(long n, const double* _
 // ---------------------------------------------------------------------------
 // Modified Gram-Schmidt sweep of one Arnoldi step in one persistent launch.
 // w stays in registers (PER elements per thread); each projection needs one
-// grid-wide exchange of block partials.  A block publishes its partial in
-// uncached memory, waits for the store's acknowledgement, then raises its
-// arrival flag to base+s+1; every block polls all flags and then folds the
-// partials in block order, so every block derives the same h_s bit for bit.
-// Partials alternate between two slots per block: a block can only overwrite
-// slot s&1 after every block has raised its flag for step s+1, i.e. after
-// every block has finished reading step s.
+// grid-wide exchange of block partials, through a ring of three slots per
+// block in uncached memory.  The value in a slot is its own arrival signal:
+// an empty slot holds the bit pattern kRingEmpty, a block publishes its
+// partial of round s into slot (r0+s) mod 3, and every block polls the G
+// slots of that round until none is empty and folds the values it read in
+// block order, so every block derives the same h_s bit for bit (the same
+// values and order as separate flag and partial reads).  Once a block has
+// read round s, every block has published round s and so finished reading
+// round s-1: the block then empties its own slot of round s-1, which is
+// written next at round s+2, and waits for that store's acknowledgement
+// before it publishes round s+1, so no reader of round s+2 can see the
+// stale value.  flags[b] carries the ring position r0 from sweep to sweep.
 // ---------------------------------------------------------------------------
 constexpr int kSweepBlock = 256;
 constexpr long kSweepSpinCap = 1L << 20;
@@ -144,20 +149,19 @@ template <int PER>
 __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
                                                           const double* __restrict__ V,
                                                           double* __restrict__ w, double* H,
-                                                          double* part, int* flags, double* err,
+                                                          double* ring, int* flags, double* err,
                                                           const int* gate, GivensArgs gv) {
   if (gate && *gate == 0) return;  // uniform: written by an earlier kernel
   __shared__ double lds[4];
   __shared__ double hsh;
   __shared__ unsigned f0s;
   const int G = gridDim.x, b = blockIdx.x;
-  // Every complete sweep raises every block's flag by j+2, so all flags are
-  // equal when a sweep starts; a block's own flag is the common base.  Flags
-  // are compared with wrap-around (serial-number) arithmetic.
+  // Every complete sweep advances every block's ring position by j+2 (mod 3),
+  // so all positions are equal when a sweep starts.
   if (threadIdx.x == 0)
     f0s = (unsigned)__hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  const unsigned f0 = f0s;
+  const int r0 = (int)(f0s % 3u);
   const long stride = (long)G * kSweepBlock;
   const long base = (long)b * kSweepBlock + threadIdx.x;
   double wr[PER], vr[PER], vn[PER];
@@ -178,6 +182,12 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
       for (int k = 0; k < PER; ++k) a += wr[k] * wr[k];
     }
     const double bs = block_sum(a, lds);
+    const int r = (r0 + s) % 3;
+    double* cur = ring + (long)r * kSweepMaxBlocks;
+    if (threadIdx.x == 0) {
+      store_ack_wait();  // the previous round's slot emptying has landed
+      __hip_atomic_store(&cur[b], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (s < j) {  // prefetch V_{s+1} while the exchange is in flight
       const double* vs = V + (long)(s + 1) * n;
 #pragma unroll
@@ -186,23 +196,19 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
         vn[k] = i < n ? vs[i] : 0.0;
       }
     }
-    const int slot = (s & 1) * kSweepMaxBlocks;
-    const unsigned target = f0 + (unsigned)(s + 1);
-    if (threadIdx.x == 0) {
-      __hip_atomic_store(&part[slot + b], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      store_ack_wait();
-      __hip_atomic_store(&flags[b], (int)target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     if (threadIdx.x < 64) {
       // bounded wait; after one expired wait the block stops waiting (the
       // host sees *err and discards the result), so a sweep always ends
       long spins = 0;
+      double t = 0.0;
       while (!broken) {
         int ok = 1;
-        for (int q = threadIdx.x; q < G; q += 64)
-          ok &= (int)((unsigned)__hip_atomic_load(&flags[q], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT) -
-                      target) >= 0;
+        t = 0.0;
+        for (int q = threadIdx.x; q < G; q += 64) {
+          const double v = __hip_atomic_load(&cur[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok &= (int)((unsigned long long)__double_as_longlong(v) != kRingEmpty);
+          t += v;
+        }
         if (__all(ok)) break;
         __builtin_amdgcn_s_sleep(1);
         if (++spins > kSweepSpinCap) {
@@ -210,11 +216,13 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
           if (threadIdx.x == 0) *err = 1.0;
         }
       }
-      double t = 0.0;
-      for (int q = threadIdx.x; q < G; q += 64)
-        t += __hip_atomic_load(&part[slot + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       t = wave_sum(t);
-      if (threadIdx.x == 0) hsh = t;
+      if (threadIdx.x == 0) {
+        hsh = t;
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(
+                               &ring[(long)((r + 2) % 3) * kSweepMaxBlocks + b]),
+                           kRingEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     __syncthreads();
     const double h = hsh;
@@ -241,9 +249,16 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
       }
     }
   }
+  // the ring position after this sweep's j+2 rounds
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&flags[b], (r0 + j + 2) % 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_fill_int(int n, int* p, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u64(int n, unsigned long long* p, unsigned long long v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
@@ -277,15 +292,19 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
   GivensArgs gv = gvp ? *gvp : GivensArgs{0.0, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (!sweep_fits<PER>(G)) return 0;
   if (wk.sweep_G != G) {
-    // the sweep needs all flags of its grid equal at entry; flags left by a
-    // different grid (or an unknown history) are reset first -- a plain
-    // kernel, so this also holds where the sequence is recorded in a graph
+    // the sweep needs all ring positions of its grid equal and the ring
+    // empty at entry; state left by a different grid (or an unknown history)
+    // is reset first -- plain kernels, so this also holds where the sequence
+    // is recorded in a graph
     hipLaunchKernelGGL(k_fill_int, dim3(4), dim3(256), 0, st, kSweepMaxBlocks, wk.flags, 0);
+    GF_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_fill_u64, dim3(12), dim3(256), 0, st, 3 * kSweepMaxBlocks,
+                       reinterpret_cast<unsigned long long*>(wk.ring), kRingEmpty);
     GF_CHECK_LAUNCH();
     wk.sweep_G = G;
   }
   if (cooperative) {
-    void* args[] = {&n,   &j,   &m,           (void*)&V,  &wv, &H, &wk.partials, &wk.flags, &err,
+    void* args[] = {&n,   &j,   &m,       (void*)&V,  &wv, &H, &wk.ring, &wk.flags, &err,
                     (void*)&gate, &gv};
     GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER>), dim3(G),
                                       dim3(kSweepBlock), args, 0, st));
@@ -294,7 +313,7 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
     // every block is resident once the kernels ahead of it drain; the waits
     // are bounded regardless
     hipLaunchKernelGGL((k_mgs_sweep<PER>), dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
-                       wk.partials, wk.flags, err, gate, gv);
+                       wk.ring, wk.flags, err, gate, gv);
     GF_CHECK_LAUNCH();
   }
   return 1;
@@ -589,6 +608,7 @@ struct Pooled {
   double* partials;
   int* ticket;
   int* flags;
+  double* ring;
 };
 struct UncachedPool {
   std::mutex mu;
@@ -640,6 +660,7 @@ int Work::init() {
     partials = e.partials;
     ticket = e.ticket;
     flags = e.flags;
+    ring = e.ring;
     sweep_G = -1;
     g_pool.free_list.pop_back();
     return 0;
@@ -651,6 +672,12 @@ int Work::init() {
   GF_HIP(hipExtMallocWithFlags((void**)&flags, sizeof(int) * kSweepMaxBlocks,
                                hipDeviceMallocUncached));
   GF_TRY(zero_now(flags, sizeof(int) * kSweepMaxBlocks));
+  GF_HIP(hipExtMallocWithFlags((void**)&ring, sizeof(double) * 3 * kSweepMaxBlocks,
+                               hipDeviceMallocUncached));
+  {
+    std::vector<unsigned long long> empty(3 * kSweepMaxBlocks, kRingEmpty);
+    GF_TRY(upload_now(ring, empty.data(), sizeof(unsigned long long) * empty.size()));
+  }
   sweep_G = -1;
   return 0;
 }
@@ -658,10 +685,11 @@ int Work::init() {
 void Work::release() {
   GF_DEVICE_LOCK();
   std::lock_guard<std::mutex> lock(g_pool.mu);
-  if (partials) g_pool.free_list.push_back(Pooled{partials, ticket, flags});
+  if (partials) g_pool.free_list.push_back(Pooled{partials, ticket, flags, ring});
   partials = nullptr;
   ticket = nullptr;
   flags = nullptr;
+  ring = nullptr;
 }
 
 }  // namespace gf

@@ -13,15 +13,18 @@ namespace gf {
 struct Work {
   double* partials = nullptr;  // 4 * kRedBlocks doubles
   int* ticket = nullptr;
-  int* flags = nullptr;        // kSweepMaxBlocks arrival flags of the MGS sweep
+  int* flags = nullptr;        // kSweepMaxBlocks ring positions of the MGS sweep
+  double* ring = nullptr;      // 3 x kSweepMaxBlocks exchange slots of the MGS sweep
   int sweep_G = -1;            // grid of the last sweep launched on flags (-1: unknown)
   int init();
   void release();
 };
 
-// Blocks of one cooperative MGS sweep (bounded by the arrival-flag array and
-// by the partials, two slots per block).
+// Blocks of one cooperative MGS sweep (bounded by the ring-position array and
+// by the exchange ring, three slots per block).
 constexpr int kSweepMaxBlocks = 1024;
+// Bit pattern of an empty exchange slot (a NaN no arithmetic produces).
+constexpr unsigned long long kRingEmpty = 0x7FF4DEADBEEFCAFEull;
 
 // Fine-operator launch shapes timed by the autotuner, in this order: 1 lane
 // per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16.
