@@ -1252,40 +1252,51 @@ int fixed_pcg_grid(Work& w, const Grid& g, const DofMap& map, const double* A,
 // The same PCG in ONE persistent launch (k_pcg_persistent): a block owns 64
 // consecutive coarsest nodes (4 waves share the 27 stencil slots as in
 // k_pcg_spmv; wave 0 keeps its nodes' x, r, p, Ap in registers).  The two
-// dot products of an iteration are grid-wide exchanges of block partials as
-// in the Gram-Schmidt sweep: a block stores its partial in uncached memory,
-// waits for the store's acknowledgement, raises its arrival flag, and every
-// block folds all partials in block order (identical bits everywhere).  The
-// vectors other blocks read (r and the two direction buffers) also live in
-// uncached memory and are written and read with agent-scope atomic accesses
-// (which bypass the CU's L1), so a store is visible to every block once
-// acknowledged -- no L2 write-back is needed.  Flags are cleared by a fill
-// kernel before every launch.  Waits are bounded: an expired wait sets *err
-// (the outer solve reports it) and the block stops waiting, so the launch
-// always ends.
+// dot products of an iteration are grid-wide exchanges of block partials
+// through a ring of three slots per block in uncached memory, as in the
+// Gram-Schmidt sweep: the value in a slot is its own arrival signal (an empty
+// slot holds the bit pattern kRingEmpty), exchange e uses slot e mod 3, and
+// every block polls the G slots of its round until none is empty and folds
+// the values in block order (identical bits everywhere).  A block that has
+// read round e empties its slot of round e-1 (every block is then past it),
+// and that store is acknowledged before the block publishes round e+1, so a
+// reader of round e+2 never sees the stale value.  The vectors other blocks
+// read (r and the two direction buffers) also live in uncached memory and are
+// written and read with agent-scope atomic accesses (which bypass the CU's
+// L1); their stores are acknowledged before the block's partial is published,
+// so they are visible to every block that has seen the partial -- no L2
+// write-back is needed.  The ring is emptied by a fill kernel before every
+// launch.  Waits are bounded: an expired wait sets *err (the outer solve
+// reports it) and the block stops waiting, so the launch always ends.
 // ---------------------------------------------------------------------------
 constexpr long kPcgSpinCap = 1L << 20;
 constexpr int kPcgMaxBlocks = 1024;
 
-// Block-sum v, publish it as exchange e, wait for every block's, fold them in
-// block order.  All threads of the block must call it.
-__device__ double pcg_exchange(double v, int e, int* flags, double* part, double* err,
-                               bool& broken, double* lds, double* bc) {
+// Block-sum v, publish it as round e of the ring, wait for every block's
+// round-e partial, fold them in block order.  All threads of the block must
+// call it.
+__device__ double pcg_exchange(double v, int e, double* ring, double* err, bool& broken,
+                               double* lds, double* bc) {
   const int G = gridDim.x, b = blockIdx.x;
   const double bs = block_sum(v, lds);
-  const int slot = (e & 1) * kPcgMaxBlocks;
-  const int target = e + 1;
+  double* cur = ring + (long)(e % 3) * kPcgMaxBlocks;
   if (threadIdx.x == 0) {
-    __hip_atomic_store(&part[slot + b], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // wave 0's vector stores of this iteration and the previous round's slot
+    // emptying have landed before the partial can be seen
     store_ack_wait();
-    __hip_atomic_store(&flags[b], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&cur[b], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (threadIdx.x < 64) {
     long spins = 0;
+    double t = 0.0;
     while (!broken) {
       int ok = 1;
-      for (int q = threadIdx.x; q < G; q += 64)
-        ok &= __hip_atomic_load(&flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+      t = 0.0;
+      for (int q = threadIdx.x; q < G; q += 64) {
+        const double pv = __hip_atomic_load(&cur[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok &= (int)((unsigned long long)__double_as_longlong(pv) != kRingEmpty);
+        t += pv;
+      }
       if (__all(ok)) break;
       __builtin_amdgcn_s_sleep(1);
       if (++spins > kPcgSpinCap) {
@@ -1293,11 +1304,13 @@ __device__ double pcg_exchange(double v, int e, int* flags, double* part, double
         if (threadIdx.x == 0) *err = 1.0;
       }
     }
-    double t = 0.0;
-    for (int q = threadIdx.x; q < G; q += 64)
-      t += __hip_atomic_load(&part[slot + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     t = wave_sum(t);
-    if (threadIdx.x == 0) *bc = t;
+    if (threadIdx.x == 0) {
+      *bc = t;
+      __hip_atomic_store(reinterpret_cast<unsigned long long*>(
+                             &ring[(long)((e + 2) % 3) * kPcgMaxBlocks + b]),
+                         kRingEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   __syncthreads();
   return *bc;
@@ -1309,9 +1322,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_persistent(Grid g, DofMap map,
                                                            const double* __restrict__ dinv,
                                                            int iters, const double* __restrict__ b,
                                                            double* __restrict__ x, double* r,
-                                                           double* p0, double* p1, int* flags,
-                                                           double* part, double* err,
-                                                           const int* gate) {
+                                                           double* p0, double* p1, double* ring,
+                                                           double* err, const int* gate) {
   if (gate && *gate == 0) return;
   __shared__ double tk[27][3][64];
   __shared__ double lds[4];
@@ -1341,16 +1353,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_persistent(Grid g, DofMap map,
     }
   }
   int e = 0;
-  double rz = pcg_exchange(a, e++, flags, part, err, broken, lds, &bc);
+  double rz = pcg_exchange(a, e++, ring, err, broken, lds, &bc);
   double rz_prev = 1.0;
   for (int it = 0; it < iters; ++it) {
     double* pn = (it & 1) ? p1 : p0;
     const double* po = (it & 1) ? p0 : p1;
     const double beta = it == 0 ? 0.0 : rz / (rz_prev + 1e-300);
-    for (int k = wv; k < 27; k += 4) {
+    // A wave's 7 slot rounds (the last masked off for wave 3) with a
+    // compile-time trip count and the LDS writes after them, so the
+    // uncached r and p loads of every round are in flight together.
+    double sv[7][3];
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      const bool kin = (t < 6) | (wv < 3);
+      const int k = kin ? wv + 4 * t : 0;
       const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
-      const bool in = valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
-                      (jz < g.NZ);
+      const bool in = kin & valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) &
+                      (jz >= 0) & (jz < g.NZ);
       const int m = (jx * g.NY + jy) * g.NZ + jz;
       double s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
@@ -1371,9 +1390,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_persistent(Grid g, DofMap map,
         s1 += A[(k * 9 + 1 * 3 + q) * nn + nn_] * xv;
         s2 += A[(k * 9 + 2 * 3 + q) * nn + nn_] * xv;
       }
-      tk[k][0][lane] = s0;
-      tk[k][1][lane] = s1;
-      tk[k][2][lane] = s2;
+      sv[t][0] = s0;
+      sv[t][1] = s1;
+      sv[t][2] = s2;
+    }
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      if ((t < 6) | (wv < 3)) {
+        const int k = wv + 4 * t;
+        tk[k][0][lane] = sv[t][0];
+        tk[k][1][lane] = sv[t][1];
+        tk[k][2][lane] = sv[t][2];
+      }
     }
     __syncthreads();
     a = 0.0;
@@ -1388,7 +1416,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_persistent(Grid g, DofMap map,
       ar[q] = acc;
       a += pr[q] * acc;
     }
-    const double pAp = pcg_exchange(a, e++, flags, part, err, broken, lds, &bc);
+    const double pAp = pcg_exchange(a, e++, ring, err, broken, lds, &bc);
     const double alpha = rz / (pAp + 1e-300);
     a = 0.0;
 #pragma unroll
@@ -1400,15 +1428,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_persistent(Grid g, DofMap map,
       a += rr[q] * __dmul_rn(dr[q], rr[q]);
     }
     rz_prev = rz;
-    rz = pcg_exchange(a, e++, flags, part, err, broken, lds, &bc);
+    rz = pcg_exchange(a, e++, ring, err, broken, lds, &bc);
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q)
     if (o[q] >= 0) x[o[q]] = xr[q];
 }
 
-__global__ void k_fill_flags(int n, int* p) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+__global__ void k_empty_ring(int n, unsigned long long* p) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = kRingEmpty;
 }
 
 // Uncached buffers of the persistent PCG: handed out by size and recycled,
@@ -1427,8 +1456,8 @@ std::vector<UcBlock>& uc_free_list() {
 
 int pcg_buffers_acquire(long n_free, void** out, size_t* bytes_out) {
   GF_DEVICE_LOCK();
-  const size_t bytes = sizeof(int) * kPcgMaxBlocks + sizeof(double) * 2 * kPcgMaxBlocks +
-                       sizeof(double) * 3 * (size_t)n_free + 256;
+  const size_t bytes = sizeof(double) * 3 * kPcgMaxBlocks + sizeof(double) * 3 * (size_t)n_free +
+                       256;
   auto& fl = uc_free_list();
   for (size_t i = 0; i < fl.size(); ++i) {
     if (fl[i].bytes >= bytes) {
@@ -1471,19 +1500,19 @@ int fixed_pcg_persistent(const Grid& g, const DofMap& map, const double* A, cons
                          double* err, const int* gate, hipStream_t st) {
   const int G = cdiv(g.n_nodes, 64);
   if (G > kPcgMaxBlocks || 4 * G > pcg_capacity()) return 0;
-  int* flags = static_cast<int*>(ucbuf);
-  double* part = reinterpret_cast<double*>(static_cast<char*>(ucbuf) + sizeof(int) * kPcgMaxBlocks);
-  double* r = part + 2 * kPcgMaxBlocks;
+  double* ring = static_cast<double*>(ucbuf);
+  double* r = ring + 3 * kPcgMaxBlocks;
   double* p0 = r + n_free;
   double* p1 = p0 + n_free;
-  hipLaunchKernelGGL(k_fill_flags, dim3(4), dim3(256), 0, st, kPcgMaxBlocks, flags);
+  hipLaunchKernelGGL(k_empty_ring, dim3(12), dim3(256), 0, st, 3 * kPcgMaxBlocks,
+                     reinterpret_cast<unsigned long long*>(ring));
   GF_CHECK_LAUNCH();
   if (map.off >= 0)
     hipLaunchKernelGGL(k_pcg_persistent<true>, dim3(G), dim3(kBlock), 0, st, g, map, A, dinv,
-                       iters, b, x, r, p0, p1, flags, part, err, gate);
+                       iters, b, x, r, p0, p1, ring, err, gate);
   else
     hipLaunchKernelGGL(k_pcg_persistent<false>, dim3(G), dim3(kBlock), 0, st, g, map, A, dinv,
-                       iters, b, x, r, p0, p1, flags, part, err, gate);
+                       iters, b, x, r, p0, p1, ring, err, gate);
   GF_CHECK_LAUNCH();
   return 1;
 }

@@ -304,7 +304,7 @@ int fixed_pcg_grid(Work& w, const Grid& g, const DofMap& map, const double* A,
 int fixed_pcg_persistent(const Grid& g, const DofMap& map, const double* A, const double* dinv,
                          int n_free, int iters, const double* b, double* x, void* ucbuf,
                          double* err, const int* gate, hipStream_t st);
-// Uncached scratch of the persistent PCG (flags, partials, r and two
+// Uncached scratch of the persistent PCG (exchange ring, r and two
 // direction buffers), recycled by size and never returned to the driver.
 int pcg_buffers_acquire(long n_free, void** out, size_t* bytes_out);
 void pcg_buffers_release(void* p, size_t bytes);
