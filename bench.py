@@ -67,13 +67,15 @@ def parse():
                          "0 keeps the runtime default)")
     ap.add_argument("--states-csv", default="")
     ap.add_argument("--form", default="policy",
-                    choices=["policy", "batched", "precision", "learned"],
+                    choices=["policy", "batched", "precision", "learned", "coarsest"],
                     help="policy: the plain FP64 guarded policy on the 80x40x20 Bernoulli "
                          "states (default line); batched: fixed-floor guarded solves of a "
                          "mixed-size batch, FP32 vs FP64 preconditioner; precision: one "
                          "filtered-sigmoid state through the policy with an FP32 vs FP64 "
                          "multigrid (use --grid 200x100x50); learned: train the floor "
-                         "network and compare learned + FP32 multigrid with the plain policy")
+                         "network and compare learned + FP32 multigrid with the plain policy; "
+                         "coarsest: the coarsest-level PCG in two-launch vs persistent form "
+                         "through preconditioner applies (use --grid 200x100x50)")
     ap.add_argument("--density-seed", type=int, default=0)
     ap.add_argument("--floors", default="1e-12,1e-3",
                     help="precision form: probe floor, then the raised ladder")
@@ -295,6 +297,62 @@ def batched_form(args):
             w.writeheader()
             w.writerows(r32 + r64)
     print(json.dumps(line))
+
+
+def coarsest_form(args):
+    """Coarsest-solve form: the fixed-count Jacobi PCG of the coarsest level in
+    its two-launch form (k_pcg_spmv + k_pcg_update per iteration, the default)
+    and its persistent form (k_pcg_persistent, one launch per solve), timed
+    through whole preconditioner applies of the canonical stack at --grid
+    (use 200x100x50: its coarsest level has 6,825 free DOFs, above
+    dense_chol_max, so it is solved by the PCG).  Elements are solid or void
+    (1e-3 density, SIMP --penal) with probability 0.65 / 0.35 from
+    --density-seed.  `value` is the persistent form's applies/s; the two
+    forms differ only in the grouping of the PCG's dot-product partials."""
+    import numpy as np
+    import torch
+
+    from gpu_fem import policy as gpol
+    from gpu_fem.presets import cantilever
+
+    dims = tuple(int(v) for v in args.grid.split("x"))
+    n_e = dims[0] * dims[1] * dims[2]
+    rng = np.random.default_rng(args.density_seed)
+    rho = np.where(rng.random(n_e) < 0.65, 1.0, 1e-3)
+    E = torch.as_tensor(1e-9 + rho ** args.penal, device="cuda")
+    res = {}
+    for persistent in (False, True):
+        comps = gpol._build_components(cantilever(*dims, load=args.load),
+                                       coarse_pcg_persistent=persistent, **gpol.CANONICAL_STACK)
+        gg = comps[7]
+        gg.setup(E)
+        n_coarsest = int(len(gg.level_free(gpol.CANONICAL_STACK["n_levels"] - 1)))
+        rhs = torch.as_tensor(np.random.default_rng(11).standard_normal(len(comps[2])),
+                              device="cuda")
+        z = gg.apply(rhs)
+        for _ in range(args.warmup):
+            z = gg.apply(rhs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            z = gg.apply(rhs)
+        torch.cuda.synchronize()
+        res[persistent] = ((time.perf_counter() - t0) / args.steps * 1e3, z.cpu().numpy())
+        print(f"{'persistent' if persistent else 'two-launch'}: {res[persistent][0]:.3f} ms "
+              f"per apply", file=sys.stderr, flush=True)
+        del comps, gg
+    a, b = res[False][1], res[True][1]
+    print(json.dumps({
+        "metric": "preconditioner applies/s (coarsest level by persistent PCG)",
+        "value": 1e3 / res[True][0], "unit": "applies/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res[True][0], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"canonical GMG preconditioner apply, cantilever "
+                               f"{dims[0]}x{dims[1]}x{dims[2]}, coarsest {n_coarsest} free DOFs",
+                   "two_launch_ms_per_apply": res[False][0],
+                   "persistent_ms_per_apply": res[True][0],
+                   "rel_diff_persistent_vs_two_launch":
+                       float(np.abs(a - b).max() / np.abs(a).max())}}))
 
 
 def precision_form(args):
@@ -557,6 +615,9 @@ def main():
         return
     if args.form == "learned":
         learned_form(args)
+        return
+    if args.form == "coarsest":
+        coarsest_form(args)
         return
     dims = tuple(int(v) for v in args.grid.split("x"))
     from gpu_fem.presets import cantilever

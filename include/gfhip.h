@@ -94,7 +94,7 @@ typedef struct gf_gmg_params {
   /* iterative coarsest solve (more free DOFs than dense_chol_max): 0 = the
    * fixed-count Jacobi PCG in two launches per iteration (default), 1 = in
    * one persistent launch with bounded grid-wide exchanges through uncached
-   * memory (13.4 vs 15.4 us per iteration at 1M elements).
+   * memory (13.3 vs 15.1 us per iteration at 1M elements).
    * The same recurrence; the dot products differ only in partial-sum
    * grouping. */
   int32_t coarse_pcg_persistent;
