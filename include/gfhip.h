@@ -5,8 +5,9 @@
  * C ABI: plain pointers and sizes.  Every pointer argument named d_* (or a
  * vector argument of a compute call) is DEVICE memory; `stream` is a
  * hipStream_t (NULL = default stream).  Every function returns 0 on success
- * and a negative code on failure (-1 HIP error, -2 bad argument, -3 not set
- * up); gf_last_error() returns a description of the last failure on the
+ * and a negative code on failure (-1 HIP error or an expired grid-wide
+ * wait, -2 bad argument, -3 not set up, -4 nothing to continue);
+ * gf_last_error() returns a description of the last failure on the
  * calling thread.  Calls on one object are not re-entrant.
  *
  * Reference interfaces replaced (paths relative to the reference repo):
@@ -24,6 +25,9 @@
  *   gf_gmg_level_*      GalerkinMatFreeGMG._K_gpu / _diag_inv_gpu (read-out for checks)
  *   gf_gmg_solve        _cupy_fgmres(A=MatrixFreeKff.matvec(., E), M=gmg.apply)
  *                                                         src/gpu_fem/multigrid_v4.py:416-534
+ *   gf_gmg_solve_continue  the kept floor's re-solve after the probe
+ *                       experiments/phase5/run_gmg_floor_detector_prospective.py:263-277, 313-351
+ *   gf_gmg_set_wait_flag  none (test hook for the wait-expiry error path)
  *   gf_dot_f64 / gf_nrm2_f64  cp.dot / cp.linalg.norm used by _cupy_fgmres / _cupy_pcg
  *   gf_cnn_*            learned floor predictor: no reference counterpart (the
  *                       reference's policy always probes, run_gmg_floor_detector_
@@ -191,10 +195,20 @@ int32_t gf_gmg_destroy(gf_gmg* g);
 /* E: n_elem FP64 moduli (device); fine_diag: n_free or NULL. Synchronises. */
 int32_t gf_gmg_setup(gf_gmg* g, const double* E, const double* fine_diag, void* stream);
 int32_t gf_gmg_info_get(gf_gmg* g, gf_gmg_info* info);
-/* out = M^{-1} rhs (the full nonlinear preconditioner). No host sync. */
+/* out = M^{-1} rhs (the full nonlinear preconditioner).  Synchronises the
+ * stream once at the end to read the wait-expiry flags: when a grid-wide
+ * wait of a Gram-Schmidt sweep (inner or root-local FGMRES) or of the
+ * persistent coarsest PCG gave up, the call returns -1, the error names the
+ * wait, and the flags, the sweep's exchange state and the recorded graph are
+ * reset (every such call, and gf_gmg_solve, reports and clears them). */
 int32_t gf_gmg_apply(gf_gmg* g, const double* rhs, double* out, void* stream);
-/* out = one base cycle (V or W) on rhs, no wrapper. */
+/* out = one base cycle (V or W) on rhs, no wrapper; wait flags as apply. */
 int32_t gf_gmg_vcycle(gf_gmg* g, const double* rhs, double* out, void* stream);
+/* No counterpart in the reference (its waits are host synchronisations that
+ * cannot expire): sets wait-expiry flag `which` (0 sweep, 1 persistent PCG)
+ * from the host, so the error path of apply / vcycle / cycle_trace / solve
+ * can be exercised without any kernel actually giving up. */
+int32_t gf_gmg_set_wait_flag(gf_gmg* g, int32_t which, void* stream);
 /* y = K_l x at level l (l = 0: fine operator with the set-up moduli). */
 int32_t gf_gmg_level_matvec(gf_gmg* g, int32_t level, const double* x, double* y, void* stream);
 /* copy the level's inverse diagonal (n_free[level] doubles) */
@@ -248,6 +262,20 @@ int32_t gf_gmg_cycle_trace(gf_gmg* g, const double* rhs, double* out, int32_t cy
 int32_t gf_gmg_solve(gf_gmg* g, const double* b, double* x, int32_t x0_given, double tol,
                      int32_t maxiter, int32_t restart, double* history, int32_t hist_cap,
                      int32_t* n_hist, int32_t* iters, int32_t* converged, void* stream);
+/* Continue the last gf_gmg_solve on this hierarchy to a larger maxiter, as if
+ * that solve had been called with it: the policy's kept floor re-solves the
+ * probe's system from x = 0 with restart >= probe iterations, and the first
+ * probe_iters Arnoldi steps of that solve are the probe's, bit for bit
+ * (run_gmg_floor_detector_prospective.py:263-277 then 313-351 re-run them).
+ * Valid when that solve started from x = 0 (x0_given = 0) with the same b
+ * pointer, tol and restart, ran one cycle that converged or stopped at its
+ * maxiter before restart, and no gf_gmg_setup came since; otherwise returns
+ * -4 and changes nothing.  x receives the iterate of the longer solve;
+ * history receives only the entries after the first solve's (iters and
+ * converged are those of the whole solve). */
+int32_t gf_gmg_solve_continue(gf_gmg* g, const double* b, double* x, double tol,
+                              int32_t maxiter, int32_t restart, double* history, int32_t hist_cap,
+                              int32_t* n_hist, int32_t* iters, int32_t* converged, void* stream);
 
 /* ---- learned floor predictor (no reference counterpart: the reference's
  * policy always runs the probe; the learned form proposes the floor and r50

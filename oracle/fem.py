@@ -149,6 +149,14 @@ def _bridge(n, name, zs):
                 loads=[Load("face", "top", magnitude=-1.0)], name=name)
 
 
+def _bracket(n, name):
+    # presets.py:363-369, 425-440: top (y = Ly) face fixed, point load
+    # (fx, fy) = (0.3, -0.7) at (1.0, 0.0, 0.25)
+    nelx, nely, nelz = n
+    return Spec(1.0, 2.0, 0.5, nelx, nely, nelz, supports=[Support("face", "top", "fixed")],
+                loads=[Load("point", (1.0, 0.0, 0.25), f=(0.3, -0.7, 0.0))], name=name)
+
+
 PRESETS = {
     # presets.py:34-40, 137-170 (cantilever family, tip point load)
     "cantilever_3d": lambda: _cantilever((24, 12, 6), "cantilever_3d"),
@@ -159,6 +167,10 @@ PRESETS = {
     # presets.py:43-55, 265-278 (bridge family, distributed top load)
     "bridge_3d": lambda: _bridge((30, 10, 5), "bridge_3d", (0.0, 0.25, 0.5)),
     "bridge_gpu_medium": lambda: _bridge((90, 30, 15), "bridge_gpu_medium", (0.0, 0.25, 0.5)),
+    # presets.py:363-369, 425-440 (bracket family)
+    "bracket_3d": lambda: _bracket((12, 24, 6), "bracket_3d"),
+    "bracket_gpu_medium": lambda: _bracket((30, 60, 15), "bracket_gpu_medium"),
+    "bracket_gpu_500k": lambda: _bracket((80, 160, 40), "bracket_gpu_500k"),
 }
 
 

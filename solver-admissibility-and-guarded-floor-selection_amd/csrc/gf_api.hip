@@ -323,7 +323,9 @@ int32_t gf_gmg_apply(gf_gmg* g, const double* rhs, double* out, void* stream) {
   CHECK_PTR(g);
   CHECK_PTR(rhs);
   CHECK_PTR(out);
-  return g->g.apply(rhs, out, S(stream));
+  const int rc = g->g.apply(rhs, out, S(stream));
+  if (rc) return rc;
+  return g->g.take_wait_error("gf_gmg_apply", S(stream));
 }
 
 int32_t gf_gmg_vcycle(gf_gmg* g, const double* rhs, double* out, void* stream) {
@@ -334,7 +336,9 @@ int32_t gf_gmg_vcycle(gf_gmg* g, const double* rhs, double* out, void* stream) {
     set_error("GalerkinMatFreeGMG.setup() must be called before apply().");
     return -3;
   }
-  return g->g.base_cycle(rhs, out, nullptr, S(stream));
+  const int rc = g->g.base_cycle(rhs, out, nullptr, S(stream));
+  if (rc) return rc;
+  return g->g.take_wait_error("gf_gmg_vcycle", S(stream));
 }
 
 int32_t gf_gmg_cycle_trace(gf_gmg* g, const double* rhs, double* out, int32_t cycle_type,
@@ -344,7 +348,23 @@ int32_t gf_gmg_cycle_trace(gf_gmg* g, const double* rhs, double* out, int32_t cy
   CHECK_PTR(rhs);
   CHECK_PTR(out);
   CHECK_PTR(visits);
-  return g->g.cycle_trace(rhs, out, cycle_type, visits, cap, n_visits, summary, S(stream));
+  const int rc =
+      g->g.cycle_trace(rhs, out, cycle_type, visits, cap, n_visits, summary, S(stream));
+  if (rc) return rc;
+  return g->g.take_wait_error("gf_gmg_cycle_trace", S(stream));
+}
+
+int32_t gf_gmg_set_wait_flag(gf_gmg* g, int32_t which, void* stream) {
+  CHECK_PTR(g);
+  if (which < 0 || which >= kWaitFlags) {
+    set_error("gf_gmg_set_wait_flag: which must be 0 (sweep) or 1 (persistent PCG)");
+    return -2;
+  }
+  const double one = 1.0;
+  GF_HIP(hipMemcpyAsync(g->g.d_err + which, &one, sizeof(double), hipMemcpyHostToDevice,
+                        S(stream)));
+  GF_HIP(hipStreamSynchronize(S(stream)));
+  return 0;
 }
 
 static int check_level(gf_gmg* g, int level, bool need_coarse) {
@@ -437,6 +457,24 @@ int32_t gf_gmg_solve(gf_gmg* g, const double* b, double* x, int32_t x0_given, do
   int nh = 0, it = 0, cv = 0;
   int rc = g->g.outer_solve(b, x, x0_given, tol, maxiter, restart, history, hist_cap, &nh, &it,
                             &cv, S(stream));
+  *n_hist = nh;
+  *iters = it;
+  *converged = cv;
+  return rc;
+}
+
+int32_t gf_gmg_solve_continue(gf_gmg* g, const double* b, double* x, double tol,
+                              int32_t maxiter, int32_t restart, double* history, int32_t hist_cap,
+                              int32_t* n_hist, int32_t* iters, int32_t* converged, void* stream) {
+  CHECK_PTR(g);
+  CHECK_PTR(b);
+  CHECK_PTR(x);
+  CHECK_PTR(n_hist);
+  CHECK_PTR(iters);
+  CHECK_PTR(converged);
+  int nh = 0, it = 0, cv = 0;
+  int rc = g->g.outer_solve(b, x, 0, tol, maxiter, restart, history, hist_cap, &nh, &it, &cv,
+                            S(stream), true);
   *n_hist = nh;
   *iters = it;
   *converged = cv;

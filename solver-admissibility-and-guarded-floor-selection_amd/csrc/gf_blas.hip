@@ -8,7 +8,9 @@
 // loops use this to stop after a breakdown without host synchronisation.
 #include "gf_reduce.h"
 
+#include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <mutex>
 
 namespace gf {
@@ -262,11 +264,24 @@ __global__ void k_fill_u64(int n, unsigned long long* p, unsigned long long v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
+int hw_queues() {
+  static std::atomic<int> q{0};
+  if (q.load() == 0) {
+    int v = 4;
+    if (const char* s = std::getenv("GPU_MAX_HW_QUEUES")) {
+      const int e = std::atoi(s);
+      if (e > 0) v = e;
+    }
+    q.store(std::min(32, std::max(1, v)));
+  }
+  return q.load();
+}
+
 namespace {
 // A sweep's blocks spin until every block of its grid has arrived, so the
 // whole grid must be resident at once.  Solver instances on other streams
-// may be running sweeps at the same moment (up to one per hardware queue,
-// four by default), so a grid may use at most a quarter of the device's
+// may be running sweeps (or persistent PCGs) at the same moment, one per
+// hardware queue, so a grid may use at most 1/hw_queues() of the device's
 // co-resident capacity; larger orthogonalisations use one launch per
 // projection.
 template <int PER>
@@ -282,7 +297,7 @@ int sweep_fits(int G) {
       return 0;
     max_blocks.store(per_cu * cus);
   }
-  return 4 * G <= max_blocks.load();
+  return (long)hw_queues() * G <= max_blocks.load();
 }
 
 template <int PER>

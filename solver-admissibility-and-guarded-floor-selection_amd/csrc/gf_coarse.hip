@@ -1478,15 +1478,17 @@ void pcg_buffers_release(void* p, size_t bytes) {
   uc_free_list().push_back(UcBlock{static_cast<char*>(p), bytes});
 }
 
-// Co-resident blocks of k_pcg_persistent on this device (the kernel's blocks
-// wait for each other, so its grid must fit at once, with room for the grids
-// of other streams: at most a quarter of the capacity, as for the sweep).
+// Co-resident blocks of the k_pcg_persistent variant that will be launched
+// (the kernel's blocks wait for each other, so its grid must fit at once,
+// with room for the waiting grids of other hardware queues: at most
+// 1/hw_queues() of the capacity, as for the sweep).
+template <bool OFF>
 static int pcg_capacity() {
   static std::atomic<int> cap{-1};
   if (cap.load() < 0) {
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<true>, kBlock, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent<OFF>, kBlock, 0) !=
             hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       return 0;
@@ -1499,7 +1501,8 @@ int fixed_pcg_persistent(const Grid& g, const DofMap& map, const double* A, cons
                          int n_free, int iters, const double* b, double* x, void* ucbuf,
                          double* err, const int* gate, hipStream_t st) {
   const int G = cdiv(g.n_nodes, 64);
-  if (G > kPcgMaxBlocks || 4 * G > pcg_capacity()) return 0;
+  const int cap = map.off >= 0 ? pcg_capacity<true>() : pcg_capacity<false>();
+  if (G > kPcgMaxBlocks || (long)hw_queues() * G > cap) return 0;
   double* ring = static_cast<double*>(ucbuf);
   double* r = ring + 3 * kPcgMaxBlocks;
   double* p0 = r + n_free;

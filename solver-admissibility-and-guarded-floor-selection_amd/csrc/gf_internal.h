@@ -26,6 +26,18 @@ constexpr int kSweepMaxBlocks = 1024;
 // Bit pattern of an empty exchange slot (a NaN no arithmetic produces).
 constexpr unsigned long long kRingEmpty = 0x7FF4DEADBEEFCAFEull;
 
+// Hardware queues of this process (GPU_MAX_HW_QUEUES, HIP's default 4,
+// clamped to [1, 32]).  Kernels whose blocks wait for each other (the
+// Gram-Schmidt sweep, the persistent coarsest PCG) may each run on a
+// different queue at the same moment, so one such grid may use at most
+// 1/hw_queues() of the device's co-resident capacity: then all of them fit
+// at once and every wait ends.
+int hw_queues();
+
+// Slots of the wait-expiry flag (two doubles per hierarchy): a grid-wide
+// wait of a Gram-Schmidt sweep, of the persistent coarsest PCG.
+enum WaitFlag { WAIT_SWEEP = 0, WAIT_PCG = 1, kWaitFlags = 2 };
+
 // Fine-operator launch shapes timed by the autotuner, in this order: 1 lane
 // per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16.
 constexpr int kFineShapes = 4;

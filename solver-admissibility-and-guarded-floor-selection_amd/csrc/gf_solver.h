@@ -122,7 +122,8 @@ struct Gmg {
   // outer solve
   Krylov outer;
   double* hs = nullptr;       // pinned host mirror for outer control
-  double* d_err = nullptr;    // set to 1 by a Gram-Schmidt sweep whose grid-wide wait expired
+  double* d_err = nullptr;    // kWaitFlags slots: set to 1 by a sweep / persistent PCG whose
+                              // grid-wide wait expired
   // scratch for power iterations
   double *pw_v = nullptr, *pw_d = nullptr, *pw_kv = nullptr, *pw_av = nullptr, *pw_s = nullptr;
   Work work;
@@ -159,7 +160,16 @@ struct Gmg {
   int base_cycle(const double* rhs, double* out, const int* gate, hipStream_t st);
   int outer_solve(const double* b, double* x, int x0_given, double tol, int maxiter, int restart,
                   double* history, int hist_cap, int* n_hist, int* iters, int* converged,
-                  hipStream_t st);
+                  hipStream_t st, bool resume = false);
+  // where the last outer solve stopped, for gf_gmg_solve_continue
+  struct Resume {
+    bool valid = false;
+    const double* b = nullptr;
+    double tol = 0.0, bnorm = 0.0;
+    int restart = 0, total = 0;
+    long n = 0;
+    bool conv = false;
+  } rs;
   void release();
 
   // internals
@@ -186,6 +196,8 @@ struct Gmg {
   int estimate_fine_lambda(hipStream_t st, double* out);
   int estimate_level_lambda(int l, hipStream_t st, double* out);
   int read_scalars(const double* dptr, double* host, int n, hipStream_t st);
+  int take_wait_error(const char* where, hipStream_t st);
+  int wait_failed(const double* flags, const char* where, hipStream_t st);
 };
 
 }  // namespace gf

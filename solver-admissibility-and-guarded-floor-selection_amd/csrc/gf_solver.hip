@@ -608,8 +608,8 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
   GF_TRY(dalloc(&pw_av, n0));
   GF_TRY(dalloc(&pw_s, 1024));
   GF_HIP(hipHostMalloc((void**)&hs, sizeof(double) * 16, hipHostMallocDefault));
-  GF_TRY(dalloc(&d_err, 1));
-  GF_TRY(zero_now(d_err, sizeof(double)));
+  GF_TRY(dalloc(&d_err, kWaitFlags));
+  GF_TRY(zero_now(d_err, sizeof(double) * kWaitFlags));
   if (p.root_enrichment_mode != 0) GF_TRY(dalloc(&re_w, n0));
   if (p.inner_krylov_steps > 0) GF_TRY(ik.alloc(p.inner_krylov_restart, n0));
   if (p.root_local_correction_mode == 2)
@@ -695,6 +695,37 @@ int Gmg::read_scalars(const double* dptr, double* host, int n, hipStream_t st) {
   GF_HIP(hipMemcpyAsync(hs, dptr, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   GF_HIP(hipStreamSynchronize(st));
   std::memcpy(host, hs, sizeof(double) * n);
+  return 0;
+}
+
+// A grid-wide wait that expired leaves its slot of d_err set (the blocks
+// stopped waiting, so the result of that launch is not valid).  The flags
+// are cleared, the exchange state of the sweep is reset (its ring positions
+// may disagree) and the recorded preconditioner graph is dropped, so the
+// next call starts clean; the error names the wait that gave up.
+int Gmg::wait_failed(const double* flags, const char* where, hipStream_t st) {
+  work.sweep_G = -1;
+  drop_graph();
+  LAUNCH(k_fill, kWaitFlags, st, (long)kWaitFlags, d_err, 0.0, (const int*)nullptr);
+  (void)hipStreamSynchronize(st);
+  std::string what;
+  if (flags[WAIT_SWEEP] != 0.0) what = "the Gram-Schmidt sweep";
+  if (flags[WAIT_PCG] != 0.0)
+    what += std::string(what.empty() ? "" : " and ") + "the persistent coarsest PCG";
+  set_error(std::string(where) + ": a grid-wide wait of " + what +
+            " exceeded its bound; the result is not valid");
+  return -1;
+}
+
+// Synchronise st and report (and clear) an expired grid-wide wait of any
+// launch issued so far; the ABI calls that run the preconditioner or the
+// coarsest PCG outside the outer solve end with this check.
+int Gmg::take_wait_error(const char* where, hipStream_t st) {
+  GF_HIP(hipMemcpyAsync(hs + 8, d_err, sizeof(double) * kWaitFlags, hipMemcpyDeviceToHost, st));
+  GF_HIP(hipStreamSynchronize(st));
+  double f[kWaitFlags];
+  std::memcpy(f, hs + 8, sizeof(f));
+  if (f[WAIT_SWEEP] != 0.0 || f[WAIT_PCG] != 0.0) return wait_failed(f, where, st);
   return 0;
 }
 
@@ -890,7 +921,7 @@ int Gmg::solve_coarsest(const double* rhs, double* out, const int* gate, hipStre
     int launched = 0;
     if (pcg_uc) {
       launched = fixed_pcg_persistent(c.g, c.map(), c.A, c.dinv, c.n_free, it, rhs, dst, pcg_uc,
-                                      d_err, gate, st);
+                                      d_err + WAIT_PCG, gate, st);
       if (launched < 0) return launched;
     }
     if (!launched)
@@ -1040,7 +1071,7 @@ int Gmg::fgmres_dev(Krylov& K, const std::function<int(const double*, double*, c
       int swept = 0;
       if (p.mgs_sweep) {  // plain launch: this sequence is recorded in a graph
         const GivensArgs gv{tol, K.cs, K.sn, K.g, K.sc, K.fl};
-        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, step, 0, &gv, st);
+        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err + WAIT_SWEEP, step, 0, &gv, st);
         if (swept < 0) return swept;
       }
       if (!swept) {
@@ -1228,6 +1259,7 @@ int Gmg::estimate_level_lambda(int l, hipStream_t st, double* out) {
 }
 
 int Gmg::setup(const double* E, const double* fine_diag, hipStream_t st) {
+  rs.valid = false;
   drop_graph();
   const long ne = (long)ctx->g.nx * ctx->g.ny * ctx->g.nz;
   const long n0 = lv[0].n_free;
@@ -1372,7 +1404,7 @@ __global__ void k_outer_cycle(int* fl, double* g, const double* sc) {
 
 int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int maxiter,
                      int restart, double* history, int hist_cap, int* n_hist, int* iters,
-                     int* converged, hipStream_t st) {
+                     int* converged, hipStream_t st, bool resume) {
   if (!is_setup) {
     set_error("GalerkinMatFreeGMG.setup() must be called before apply().");
     return -3;
@@ -1382,7 +1414,19 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
     return -2;
   }
   const long n = lv[0].n_free;
-  GF_TRY(outer.alloc(restart, n));
+  if (resume) {
+    if (!rs.valid || rs.b != b || rs.tol != tol || rs.restart != restart || rs.n != n ||
+        maxiter < rs.total) {
+      set_error("gf_gmg_solve_continue: no resumable solve (the last gf_gmg_solve on this "
+                "hierarchy must have started from x = 0, with the same b, tol and restart, and "
+                "stopped at its maxiter inside its first cycle or converged there; no set-up "
+                "since, and maxiter not below the iterations already done)");
+      return -4;
+    }
+  } else {
+    rs.valid = false;
+    GF_TRY(outer.alloc(restart, n));
+  }
   Krylov& K = outer;
   const int m = K.m;
   int nh = 0;
@@ -1390,11 +1434,17 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
     if (history && nh < hist_cap) history[nh] = v;
     ++nh;
   };
-  if (!x0_given) LAUNCH(k_fill, n, st, n, x, 0.0, (const int*)nullptr);
+  // a continued solve is the same first cycle, from x = 0
+  if (!x0_given || resume) LAUNCH(k_fill, n, st, n, x, 0.0, (const int*)nullptr);
   double h[4];
-  GF_TRY(dot_f64(work, n, b, b, K.sc, true, nullptr, st));
-  GF_TRY(read_scalars(K.sc, h, 1, st));
-  const double bnorm = h[0];
+  double bnorm;
+  if (resume) {
+    bnorm = rs.bnorm;
+  } else {
+    GF_TRY(dot_f64(work, n, b, b, K.sc, true, nullptr, st));
+    GF_TRY(read_scalars(K.sc, h, 1, st));
+    bnorm = h[0];
+  }
   *iters = 0;
   *converged = 0;
   if (bnorm <= 1e-30) {
@@ -1402,24 +1452,36 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
     *n_hist = 0;
     return 0;
   }
-  int total = 0;
-  bool conv = false;
-  while (total < maxiter) {
-    EpiArgs<double> ep;
-    ep.b = b;
-    GF_TRY(level_op(0, x, K.V, EPI_RESID, ep, nullptr, st));
-    GF_TRY(dot_f64(work, n, K.V, K.V, K.sc + 1, true, nullptr, st));
-    GF_TRY(read_scalars(K.sc + 1, h, 1, st));
-    const double beta = h[0];
-    push(beta / bnorm);
-    if (beta / bnorm <= tol) {
-      conv = true;
-      break;
+  int total = resume ? rs.total : 0;
+  bool conv = resume && rs.conv;
+  int j0 = resume ? rs.total : 0;  // Arnoldi step to continue from (first cycle)
+  int cycles = 0;
+  bool cut_in_cycle = false;       // the last cycle stopped at maxiter before restart
+  if (resume && conv) {
+    // the first solve converged: its iterate, recombined from the same
+    // Hessenberg solution and basis
+    GF_TRY(combine(n, K.Z, n, K.y, K.fl + 2, true, x, nullptr, st));
+    cycles = 1;
+  }
+  while (!conv && total < maxiter) {
+    const int cycle_start = total - j0;
+    if (j0 == 0) {
+      EpiArgs<double> ep;
+      ep.b = b;
+      GF_TRY(level_op(0, x, K.V, EPI_RESID, ep, nullptr, st));
+      GF_TRY(dot_f64(work, n, K.V, K.V, K.sc + 1, true, nullptr, st));
+      GF_TRY(read_scalars(K.sc + 1, h, 1, st));
+      const double beta = h[0];
+      push(beta / bnorm);
+      if (beta / bnorm <= tol) {
+        conv = true;
+        break;
+      }
+      LAUNCH1(k_outer_cycle, st, K.fl, K.g, K.sc);
+      GF_TRY(scale_inv(n, K.V, K.sc + 1, -1.0, K.V, nullptr, st));
     }
-    LAUNCH1(k_outer_cycle, st, K.fl, K.g, K.sc);
-    GF_TRY(scale_inv(n, K.V, K.sc + 1, -1.0, K.V, nullptr, st));
-    const int inner = std::min(restart, maxiter - total);
-    for (int j = 0; j < inner; ++j) {
+    const int inner = std::min(restart, maxiter - cycle_start);
+    for (int j = j0; j < inner; ++j) {
       double* Vj = K.V + (long)j * n;
       double* Zj = K.Z + (long)j * n;
       double* w = K.V + (long)(j + 1) * n;
@@ -1428,7 +1490,8 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
       int swept = 0;
       if (p.mgs_sweep) {
         const GivensArgs gv{tol, K.cs, K.sn, K.g, K.sc, K.fl};
-        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err, nullptr, p.mgs_sweep == 2, &gv, st);
+        swept = mgs_sweep(work, n, K.V, j, m, w, K.H, d_err + WAIT_SWEEP, nullptr,
+                          p.mgs_sweep == 2, &gv, st);
         if (swept < 0) return swept;
       }
       if (!swept) {
@@ -1442,18 +1505,15 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
         GF_TRY(scale_inv(n, w, K.H + (long)(j + 1) * m + j, 1e-14, w, nullptr, st));
       }
       if (!swept) LAUNCH1(k_givens, st, m, j, tol, K.H, K.cs, K.sn, K.g, K.sc, K.fl);
-      // projected residual, h_{j+1,j} and the sweep error flag in one wait
+      // projected residual, h_{j+1,j} and the wait-expiry flags in one wait
       GF_HIP(hipMemcpyAsync(hs, K.sc + 2, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
-      GF_HIP(hipMemcpyAsync(hs + 2, d_err, sizeof(double), hipMemcpyDeviceToHost, st));
+      GF_HIP(hipMemcpyAsync(hs + 2, d_err, sizeof(double) * kWaitFlags, hipMemcpyDeviceToHost,
+                            st));
       GF_HIP(hipStreamSynchronize(st));
-      std::memcpy(h, hs, sizeof(double) * 3);
-      if (h[2] != 0.0) {
-        // flags may now disagree: force a reset before the next sweep
-        work.sweep_G = -1;
-        drop_graph();
-        (void)hipMemsetAsync(d_err, 0, sizeof(double), st);
-        set_error("gf_gmg_solve: a grid-wide wait of the Gram-Schmidt sweep exceeded its bound");
-        return -1;
+      std::memcpy(h, hs, sizeof(double) * 2);
+      if (hs[2 + WAIT_SWEEP] != 0.0 || hs[2 + WAIT_PCG] != 0.0) {
+        rs.valid = false;
+        return wait_failed(hs + 2, "gf_gmg_solve", st);
       }
       ++total;
       push(h[0]);
@@ -1462,12 +1522,27 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
         break;
       }
     }
+    cut_in_cycle = !conv && (total - cycle_start) < restart;
+    j0 = 0;
+    ++cycles;
     hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(256), 0, st, m, K.H, K.g, K.y, K.fl);
     GF_CHECK_LAUNCH();
     GF_TRY(combine(n, K.Z, n, K.y, K.fl + 2, true, x, nullptr, st));
     if (conv) break;
   }
   GF_HIP(hipStreamSynchronize(st));
+  // resumable: one cycle from x = 0 that converged or stopped at maxiter
+  // before its restart length (V, Z, H, the Givens rotations and g then hold
+  // exactly the state a longer solve has at that step)
+  const bool zero_start = resume || !x0_given;
+  rs.valid = zero_start && cycles == 1 && (conv || cut_in_cycle);
+  rs.b = b;
+  rs.tol = tol;
+  rs.restart = restart;
+  rs.n = n;
+  rs.total = total;
+  rs.conv = conv;
+  rs.bnorm = bnorm;
   *iters = total;
   *converged = conv ? 1 : 0;
   *n_hist = nh;

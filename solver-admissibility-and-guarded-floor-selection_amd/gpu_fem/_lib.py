@@ -127,6 +127,9 @@ SIGNATURES = {
     "gf_gmg_cycle_trace": [_P, _P, _P, _I, C.POINTER(gf_trace_visit), _I, C.POINTER(_I), _P, _P],
     "gf_gmg_solve": [_P, _P, _P, _I, _D, _I, _I, _P, _I, C.POINTER(_I), C.POINTER(_I),
                      C.POINTER(_I), _P],
+    "gf_gmg_solve_continue": [_P, _P, _P, _D, _I, _I, _P, _I, C.POINTER(_I), C.POINTER(_I),
+                              C.POINTER(_I), _P],
+    "gf_gmg_set_wait_flag": [_P, _I, _P],
     "gf_cnn_sizes": [C.POINTER(_I), C.POINTER(C.c_int64)],
     "gf_cnn_pool_f64": [_P, _I, _I, _I, _P, _P],
     "gf_cnn_forward": [_P, _P, _P, _I, _P, _P, _P],

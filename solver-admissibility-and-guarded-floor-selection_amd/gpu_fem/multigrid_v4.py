@@ -733,6 +733,43 @@ class GalerkinMatFreeGMG:
             history.extend(float(v) for v in hist[: min(nh.value, cap)])
         return x, int(it.value), bool(cv.value)
 
+    def solve_continue(self, b, *, tol=1e-6, maxiter=200, restart=32, history=None):
+        """Continue the last ``solve`` on this hierarchy to ``maxiter`` as if it
+        had been called with that limit (gf_gmg_solve_continue).  Valid when
+        that solve started from x = 0 with the same ``b`` tensor, ``tol`` and
+        ``restart``, converged or stopped at its maxiter inside its first
+        restart cycle, and no ``setup`` came since; otherwise raises
+        ``ContinueUnavailable``.  ``history`` receives only the new entries."""
+        import torch
+
+        if not self._is_setup:
+            raise RuntimeError("GalerkinMatFreeGMG.setup() must be called before apply().")
+        bb = _as_dev(b, torch.float64)
+        x = torch.empty_like(bb)
+        cap = int(maxiter) + int(maxiter) // max(1, int(restart)) + 2
+        hist = np.zeros(cap, dtype=np.float64)
+        nh, it, cv = C.c_int32(), C.c_int32(), C.c_int32()
+        rc = self._lib.gf_gmg_solve_continue(self._h, _lib.ptr(bb), _lib.ptr(x), float(tol),
+                                             int(maxiter), int(restart), hist.ctypes.data, cap,
+                                             C.byref(nh), C.byref(it), C.byref(cv),
+                                             _lib.stream_handle())
+        if rc == -4:
+            raise ContinueUnavailable(self._lib.gf_last_error().decode())
+        _lib.check(rc, "gf_gmg_solve_continue")
+        if history is not None:
+            history.extend(float(v) for v in hist[: min(nh.value, cap)])
+        return x, int(it.value), bool(cv.value)
+
+    def _set_wait_flag(self, which: int) -> None:
+        """Raise wait-expiry flag ``which`` (0 sweep, 1 persistent PCG) from
+        the host: the next apply / vcycle / cycle_trace / solve reports it."""
+        _lib.check(self._lib.gf_gmg_set_wait_flag(self._h, int(which), _lib.stream_handle()),
+                   "gf_gmg_set_wait_flag")
+
+
+class ContinueUnavailable(RuntimeError):
+    """The last solve of a hierarchy cannot be continued (gf_gmg_solve_continue)."""
+
 
 def _native_pair(A_op, M_op):
     gmg = getattr(M_op, "__self__", None)

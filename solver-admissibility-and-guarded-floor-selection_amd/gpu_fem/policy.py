@@ -134,7 +134,7 @@ class GuardedFloorPolicy:
                  maxiter=300, tol=1e-6, high_residual_threshold=1e-2,
                  plateau_residual_threshold=1e-4, plateau_ratio_threshold=0.6,
                  severity_jump_r50_threshold=0.0, severity_jump_rho_min=1e-2,
-                 stack_variant="canonical", preset_name=None, **stack):
+                 stack_variant="canonical", preset_name=None, reuse_probe=True, **stack):
         st = dict(CANONICAL_STACK)
         st.update(stack)
         self.stack = st
@@ -155,6 +155,9 @@ class GuardedFloorPolicy:
                         plateau_ratio_threshold=float(plateau_ratio_threshold))
         self.severity_jump_r50_threshold = float(severity_jump_r50_threshold)
         self.severity_jump_rho_min = float(severity_jump_rho_min)
+        # a kept baseline floor continues the probe's Arnoldi cycle instead of
+        # repeating its set-up and first probe_iters steps (identical result)
+        self.reuse_probe = bool(reuse_probe)
         self.b_norm = nrm2(self.F)
         self.n_elem = self.spec.nelx * self.spec.nely * self.spec.nelz
 
@@ -204,13 +207,40 @@ class GuardedFloorPolicy:
                    "gpu_used_mb_after_solve": _mem_used_mb()}
         return E, x, iters, bool(reported), final <= self.tol, final, metrics
 
+    def continue_probe(self, E, probe_hist, *, maxiter, history):
+        """The kept baseline floor's solve as the continuation of the probe.
+
+        The reference solves the kept floor afresh (set-up at the same moduli,
+        FGMRES from x = 0 with restart >= probe_iters: prospective:263-277 then
+        313-351), so its first probe_iters Arnoldi steps repeat the probe's.
+        Every kernel here is deterministic and the set-up at the same moduli
+        reproduces the same hierarchy, so continuing the probe's cycle to
+        ``maxiter`` gives the from-scratch solve bit for bit (history, iterate,
+        iteration count) without repeating it.  Raises ContinueUnavailable
+        when the probe cannot be continued."""
+        t1 = time.perf_counter()
+        new: list = []
+        x, iters, reported = self.gmg.solve_continue(self.F, tol=self.tol, maxiter=maxiter,
+                                                     restart=self.restart, history=new)
+        _sync()
+        solve_time = time.perf_counter() - t1
+        history.extend(probe_hist)
+        history.extend(new)
+        r = self.mf_op.residual(self.F, x, E)
+        final = nrm2(r) / self.b_norm
+        metrics = {"setup_time_s": 0.0, "solve_time_s": solve_time,
+                   "gpu_used_mb_before": float("nan"), "gpu_used_mb_after_setup": float("nan"),
+                   "gpu_used_mb_after_solve": _mem_used_mb(), "continued_probe": True}
+        return E, x, iters, bool(reported), final <= self.tol, final, metrics
+
     def run_state(self, solid=None, *, rho_field=None, seed=None, probability=None,
                   with_connectivity=False, keep_x=True):
         """Probe -> classify -> ladder -> guard -> fallback for one state."""
+        from .multigrid_v4 import ContinueUnavailable
         from .solver_v2 import dot
 
         probe_hist: list = []
-        _, _, _, _, _, _, probe_m = self.solve_at_floor(
+        probe_E, _, _, _, _, _, probe_m = self.solve_at_floor(
             self.baseline_rho_min, maxiter=self.probe_iters, history=probe_hist, solid=solid,
             rho_field=rho_field)
         r50 = probe_hist[min(self.probe_r50, len(probe_hist) - 1)]
@@ -233,8 +263,18 @@ class GuardedFloorPolicy:
                     continue
                 st["attempted"].append(rho_min)
                 h: list = []
-                E, x, it, rep, ok, fin, m = self.solve_at_floor(
-                    rho_min, maxiter=self.maxiter, history=h, solid=solid, rho_field=rho_field)
+                res = None
+                if (self.reuse_probe and len(st["attempted"]) == 1
+                        and rho_min == self.baseline_rho_min):
+                    try:
+                        res = self.continue_probe(probe_E, probe_hist, maxiter=self.maxiter,
+                                                  history=h)
+                    except ContinueUnavailable:
+                        res = None
+                if res is None:
+                    res = self.solve_at_floor(rho_min, maxiter=self.maxiter, history=h,
+                                              solid=solid, rho_field=rho_field)
+                E, x, it, rep, ok, fin, m = res
                 st["hist"].append((rho_min, h))
                 st.update(rho=rho_min, iters=it, ok=ok, reported=rep, final=fin, x=x, E=E, sel=m)
                 if ok:
@@ -287,6 +327,7 @@ class GuardedFloorPolicy:
                                        + st["fail_setup"] + st["fail_solve"]
                                        + sel["setup_time_s"] + sel["solve_time_s"]),
             "compliance": compliance,
+            "selected_continued_probe": int(bool(sel.get("continued_probe", False))),
         }
         if with_connectivity and solid is not None:
             m = np.asarray(solid).reshape(self.spec.nelx, self.spec.nely, self.spec.nelz)

@@ -1,5 +1,6 @@
 """Named problem presets used by the floor-policy workloads (presets.py of the
-reference: cantilever family 34-40 / 137-170, bridge family 43-55 / 265-278)."""
+reference: cantilever family 34-40 / 137-170, bridge family 43-55 / 265-278,
+bracket family 363-369 / 425-440)."""
 
 from __future__ import annotations
 
@@ -21,6 +22,12 @@ def _bridge(nelx, nely, nelz, zs):
                        supports=sup, loads=[DistributedLoad(edge="top", magnitude=-1.0)])
 
 
+def _bracket(nelx, nely, nelz):
+    return ProblemSpec(Lx=1.0, Ly=2.0, Lz=0.5, nelx=nelx, nely=nely, nelz=nelz, volfrac=0.3,
+                       supports=[EdgeSupport(edge="top", constraint="fixed")],
+                       loads=[PointLoad(x=1.0, y=0.0, z=0.25, fy=-0.7, fx=0.3)])
+
+
 _PRESETS = {
     "cantilever_3d": lambda: _cantilever(24, 12, 6),
     "cantilever_3d_large": lambda: _cantilever(40, 20, 10),
@@ -31,6 +38,10 @@ _PRESETS = {
     "bridge_3d": lambda: _bridge(30, 10, 5, (0.0, 0.25, 0.5)),
     "bridge_gpu_medium": lambda: _bridge(90, 30, 15, (0.0, 0.25, 0.5)),
     "bridge_gpu_large": lambda: _bridge(150, 50, 25, (0.0, 0.125, 0.25, 0.375, 0.5)),
+    "bracket_3d": lambda: _bracket(12, 24, 6),
+    "bracket_gpu_medium": lambda: _bracket(30, 60, 15),
+    "bracket_gpu_500k": lambda: _bracket(80, 160, 40),
+    "bracket_gpu_1M": lambda: _bracket(100, 200, 50),
 }
 
 

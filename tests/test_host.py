@@ -70,11 +70,21 @@ def test_host_setup_equals_oracle():
         assert grid_from_edof(ed) == dims
     with pytest.raises(ValueError):
         grid_from_edof(_edof_table_3d(3, 2, 2)[::-1])
-    for name in ["cantilever_gpu_medium", "bridge_gpu_medium", "cantilever_3d"]:
+    for name in ["cantilever_gpu_medium", "bridge_gpu_medium", "cantilever_3d", "bracket_3d",
+                 "bracket_gpu_medium"]:
         a = bc_generator.generate_bc(presets.get_preset(name))
         b = fem.generate_bc(fem.get_preset(name))
         assert np.array_equal(a.free_dofs, b.free_dofs)
         assert np.array_equal(a.F, b.F)
+    # bracket (presets.py:363-369): the y = Ly face is clamped, and the point
+    # load (fx, fy) = (0.3, -0.7) lands on node (nelx, 0, nelz/2)
+    nx, ny, nz = 30, 60, 15
+    b = fem.generate_bc(fem.get_preset("bracket_gpu_medium"))
+    assert b.ndof - len(b.free_dofs) == 3 * (nx + 1) * (nz + 1)
+    node = (nx * (ny + 1) + 0) * (nz + 1) + nz // 2 + (nz % 2)
+    nz_idx = np.nonzero(b.F)[0]
+    assert set(nz_idx) <= {3 * node, 3 * node + 1, 3 * (node - 1), 3 * (node - 1) + 1}
+    assert np.isclose(b.F.sum(), -0.4) and np.isclose(b.F[nz_idx[0]], 0.3)
     a = bc_generator.generate_bc(presets.cantilever(40, 20, 20, load="edge"))
     b = fem.generate_bc(fem.cantilever_spec(40, 20, 20, load="edge"))
     assert np.array_equal(a.F, b.F) and np.array_equal(a.free_dofs, b.free_dofs)
