@@ -79,6 +79,26 @@ def main(ref_root):
         w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
         w.writeheader()
         w.writerows(rows)
+    # 512k-element optimized densities of the fixed-floor controls (cantilever
+    # 160x80x40 and bracket 80x160x40): the arrays, compressed, with the
+    # reference manifest's SHA-256 of the original .npy file
+    import hashlib
+
+    import numpy as np
+
+    man = {}
+    with open(os.path.join(ref_root, "data", "optimized_density_inputs", "MANIFEST.csv"),
+              newline="") as f:
+        for r in csv.DictReader(f):
+            man[r["file"]] = r["sha256"]
+    for case in ("C512_MF", "Brk500_MF"):
+        name = f"{case}__rho_final.npy"
+        path = os.path.join(ref_root, "data", "optimized_density_inputs", name)
+        raw = open(path, "rb").read()
+        assert hashlib.sha256(raw).hexdigest() == man[name], name
+        rho = np.load(path, allow_pickle=False)
+        np.savez_compressed(os.path.join(out, f"{case}__rho_final.npz"), rho=rho,
+                            npy_sha256=np.array(man[name]))
 
 
 if __name__ == "__main__":
