@@ -184,6 +184,18 @@ int32_t gf_nodeblock_apply(gf_ctx* ctx, const double* r, double* z, void* stream
 int32_t gf_dot_f64(gf_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out,
                    void* stream);
 int32_t gf_nrm2_f64(gf_ctx* ctx, int64_t n, const double* x, double* d_out, void* stream);
+/* The recomputed-residual guard's two norms without any allocation:
+ * d_out[0] = ||b - K_ff(E) u||_2, d_out[1] = ||b||_2 (device, 2 doubles),
+ * bitwise what gf_residual_f64 followed by gf_nrm2_f64 on the residual and
+ * on b give with this context.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc): the residual and the
+ * partial sums live in a file-scope device array sized for at most 4,194,304
+ * free DOFs (32 MiB, held by every GPU the library runs on until the process
+ * ends); a larger context returns -2.  Calls on one GPU share that array, so
+ * calls that overlap on one GPU race.  No host sync. */
+int32_t gf_residual_norm_f64_noalloc(gf_ctx* ctx, const double* E, const double* b,
+                                     const double* u, double* d_out, void* stream);
 /* y = a*x + b*y (host scalars); the vector updates of _cupy_fgmres/_cupy_pcg
  * when they run with caller-supplied operators */
 int32_t gf_axpby_f64(int64_t n, double a, const double* x, double b, double* y, void* stream);

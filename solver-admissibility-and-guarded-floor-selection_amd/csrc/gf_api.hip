@@ -1,6 +1,8 @@
 // extern "C" entry points (declared in include/gfhip.h).
+#include "gf_reduce.h"
 #include "gf_solver.h"
 
+#include <mutex>
 #include <new>
 
 using namespace gf;
@@ -21,6 +23,64 @@ static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); 
       return -2;                                      \
     }                                                 \
   } while (0)
+
+// ---------------------------------------------------------------------------
+// Recomputed-residual guard without allocation (gf_residual_norm_f64_noalloc).
+// Device scratch of a fixed maximum size in a file-scope array: the residual
+// (kNoAllocMaxFree doubles) and the per-block partial sums (kRedBlocks).
+// Every GPU the library runs on holds its own copy until the process ends;
+// host code uses the address hipGetSymbolAddress returns for the current
+// device, never the array's name.  Calls on one GPU share that GPU's copy,
+// so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long kNoAllocMaxFree = 1L << 22;  // 4,194,304 free DOFs (32 MiB)
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocMaxFree + kRedBlocks];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+static int scratch_ptr(double** out) {
+  int dev = 0;
+  GF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error("gf_residual_norm_f64_noalloc: device ordinal out of range");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("gf_residual_norm_f64_noalloc: hipGetSymbolAddress: ") +
+                hipGetErrorString(e));
+      return -1;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// Block partials of x.x: the loop and block tree of k_dot (same bits).
+__global__ __launch_bounds__(kBlock) void k_sq_partials(long n, const double* __restrict__ x,
+                                                        double* __restrict__ partials) {
+  __shared__ double lds[4];
+  const double bs = block_sum(dot_thread<double>(n, x, x), lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// sqrt of the partials folded in block order: the fold of k_dot's last block.
+__global__ __launch_bounds__(kBlock) void k_fold_sqrt(int nb, const double* __restrict__ partials,
+                                                      double* __restrict__ out) {
+  __shared__ double lds[4];
+  const double s = fold_thread_first(nb, partials, lds);
+  if (threadIdx.x == 0) *out = sqrt(s);
+}
 
 extern "C" {
 
@@ -206,6 +266,41 @@ int32_t gf_nrm2_f64(gf_ctx* ctx, int64_t n, const double* x, double* d_out, void
   Work* w = blas_work(ctx);
   if (!w) return -1;
   return dot_f64(*w, n, x, x, d_out, true, nullptr, S(stream));
+}
+
+int32_t gf_residual_norm_f64_noalloc(gf_ctx* ctx, const double* E, const double* b,
+                                     const double* u, double* d_out, void* stream) {
+  CHECK_PTR(ctx);
+  CHECK_PTR(E);
+  CHECK_PTR(b);
+  CHECK_PTR(u);
+  CHECK_PTR(d_out);
+  const long n = ctx->c.n_free;
+  if (n > kNoAllocMaxFree) {
+    set_error("gf_residual_norm_f64_noalloc: " + std::to_string(n) +
+              " free DOFs exceed the scratch maximum of " + std::to_string(kNoAllocMaxFree));
+    return -2;
+  }
+  double* base = nullptr;
+  GF_TRY(scratch_ptr(&base));
+  double* r = base;
+  double* partials = base + kNoAllocMaxFree;
+  const hipStream_t st = S(stream);
+  EpiArgs<double> ep;
+  ep.b = b;
+  GF_TRY(fine_matvec<double>(ctx->c.g, ctx->c.map(), ctx->c.Ke, E, u, r, EPI_RESID, ep, nullptr,
+                             st));
+  // every partial the fold reads is written by the kernel before it on this
+  // stream, so nothing in the scratch has to start at zero
+  const int nb = red_grid(n);
+  const double* vec[2] = {r, b};
+  for (int k = 0; k < 2; ++k) {
+    hipLaunchKernelGGL(k_sq_partials, dim3(nb), dim3(kBlock), 0, st, n, vec[k], partials);
+    GF_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_fold_sqrt, dim3(1), dim3(kBlock), 0, st, nb, partials, d_out + k);
+    GF_CHECK_LAUNCH();
+  }
+  return 0;
 }
 
 __global__ void k_axpby_host(long n, double a, const double* __restrict__ x, double b,

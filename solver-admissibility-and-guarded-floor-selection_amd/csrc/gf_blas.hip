@@ -31,22 +31,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(long n, const T* __restrict__ x,
                                                 int* ticket, double* out, int post,
                                                 const int* gate) {
   if (gate && *gate == 0) return;
-  double a[1] = {0.0};
-  // four strides' loads issue before the first is used; the sum is taken
-  // in the same order as the plain stride loop (identical bits)
-  const long st = (long)gridDim.x * blockDim.x;
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  for (; i + 3 * st < n; i += 4 * st) {
-    T xv[4], yv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      xv[u] = x[i + u * st];
-      yv[u] = y[i + u * st];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) a[0] += (double)xv[u] * (double)yv[u];
-  }
-  for (; i < n; i += st) a[0] += (double)x[i] * (double)y[i];
+  double a[1] = {dot_thread<T>(n, x, y)};
   finish_reduce<1>(a, partials, ticket, out, post);
 }
 

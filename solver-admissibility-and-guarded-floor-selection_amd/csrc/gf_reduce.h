@@ -29,6 +29,37 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return r;
 }
 
+// One thread's share of sum_i x_i y_i over a grid-stride loop: four
+// strides' loads issue before the first is used, and the sum is taken in the
+// same order as the plain stride loop (identical bits).
+template <typename T>
+__device__ __forceinline__ double dot_thread(long n, const T* __restrict__ x,
+                                             const T* __restrict__ y) {
+  double a = 0.0;
+  const long st = (long)gridDim.x * blockDim.x;
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    T xv[4], yv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xv[u] = x[i + u * st];
+      yv[u] = y[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a += (double)xv[u] * (double)yv[u];
+  }
+  for (; i < n; i += st) a += (double)x[i] * (double)y[i];
+  return a;
+}
+
+// Sum of nb block partials in block order over one 256-thread block (each
+// thread takes i, i + 256, ..., then the block tree); result in thread 0.
+__device__ __forceinline__ double fold_thread_first(int nb, const double* partials, double* lds) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) a += partials[i];
+  return block_sum(a, lds);
+}
+
 // Fold NP partial sums of every block into out[0..NP) (last block only).
 // Returns true in the block that performed the fold.
 template <int NP>

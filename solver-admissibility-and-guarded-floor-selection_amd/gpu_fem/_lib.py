@@ -111,6 +111,7 @@ SIGNATURES = {
     "gf_dot_f64": [_P, C.c_int64, _P, _P, _P, _P],
     "gf_nrm2_f64": [_P, C.c_int64, _P, _P, _P],
     "gf_axpby_f64": [C.c_int64, _D, _P, _D, _P, _P],
+    "gf_residual_norm_f64_noalloc": [_P, _P, _P, _P, _P, _P],
     "gf_gmg_default_params": [C.POINTER(gf_gmg_params)],
     "gf_gmg_create": [C.POINTER(_P), _P, C.POINTER(gf_gmg_params)],
     "gf_gmg_destroy": [_P],
