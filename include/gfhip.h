@@ -136,13 +136,15 @@ int32_t gf_ctx_destroy(gf_ctx* ctx);
 /* Launch-shape autotuner of the fine operator (no reference counterpart):
  * times the fine-operator kernels on this grid -- shape 1 one lane per node,
  * 4 four waves per 64 nodes, 8 and 16 the same with the node and element
- * windows staged in LDS for an 8x8 / 4x16 (y,z) patch -- and keeps the
- * fastest for every later call on ctx (each shape's best of three
+ * windows staged in LDS for an 8x8 / 4x16 (y,z) patch, 72 and 80 those tiles
+ * with the FP64 element-row contraction on the matrix cores
+ * (v_mfma_f64_4x4x4f64; the FP32 operator runs the VALU tile there) -- and
+ * keeps the fastest for every later call on ctx (each shape's best of three
  * interleaved timing rounds).  All shapes give bitwise-identical
- * results.  us_out (nullable, host): 4 floats of timings in that order;
+ * results.  us_out (nullable, host): 6 floats of timings in that order;
  * shape_out (nullable, host): the chosen shape.  Synchronises. */
 int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* stream);
-/* Force a launch shape: 0 = process default, 1, 4, 8 or 16. */
+/* Force a launch shape: 0 = process default, 1, 4, 8, 16, 72 or 80. */
 int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape);
 /* Process-wide launch form of the coarse-level (27-point block stencil)
  * operator kernels: 1 = one lane per node, slot groups across four waves,

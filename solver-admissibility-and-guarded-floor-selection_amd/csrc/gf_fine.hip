@@ -12,6 +12,7 @@
 #include "gf_internal.h"
 
 #include <atomic>
+#include <type_traits>
 
 namespace gf {
 
@@ -207,39 +208,21 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
   epilogue3<T, EPI>(o, ax, u, y, ep);
 }
 
-// LDS-tiled form of the four-wave kernel.  A block covers a TY x TZ patch of
-// nodes in one x-plane (64 nodes) and first stages, with branch-free masked
-// loads, the u values of the 3 x (TY+2) x (TZ+2) node window and the moduli
-// of the 2 x (TY+1) x (TZ+1) element window in LDS (zeros outside the grid
-// and on fixed DOFs).  Wave w then forms the element pair (a,b) = (w>>1, w&1)
-// exactly as k_fine_matvec_wave4 does, reading its window from LDS, and the
-// pair sums meet in the same fixed tree, so the result is bitwise identical.
-// One global read per window entry replaces the 36-per-lane gathers.
-template <typename T, int EPI, bool OFF, int TZ>
-__global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
-                                                             const T* __restrict__ Ke,
-                                                             const T* __restrict__ E,
-                                                             const T* __restrict__ u,
-                                                             T* __restrict__ y, EpiArgs<T> ep,
-                                                             const int* gate) {
-  using A = typename Acc<T>::type;
+// Stage the window of the (ix, y0.., z0..) node patch of an LDS tile kernel,
+// with branch-free masked loads: us entry ((xx*WY + yy)*WZ + zz)*3 + q = node
+// (ix-1+xx, y0-1+yy, z0-1+zz) component q (0 outside the grid and on fixed
+// DOFs), es entry (ex*EY + ey)*EZ + ez = element (ix-1+ex, y0-1+ey, z0-1+ez)
+// (0 outside).  Every load of a thread is issued before the first LDS write
+// (masked loads read a valid address), so they are in flight together.
+// Ends with a barrier.
+template <typename T, bool OFF, int TZ>
+__device__ __forceinline__ void stage_window(const Grid& g, const DofMap& map,
+                                             const T* __restrict__ E, const T* __restrict__ u,
+                                             T* us, T* es, int ix, int y0, int z0) {
   constexpr int TY = 64 / TZ;
   constexpr int WY = TY + 2, WZ = TZ + 2;      // node window (x: 3 planes)
   constexpr int EY = TY + 1, EZ = TZ + 1;      // element window (x: 2 planes)
   constexpr int NU = 3 * WY * WZ * 3, NE = 2 * EY * EZ;
-  if (gate && *gate == 0) return;
-  __shared__ T us[NU];
-  __shared__ T es[NE];
-  __shared__ A part[3][3][64];
-  const int tiles_z = (g.NZ + TZ - 1) / TZ, tiles_y = (g.NY + TY - 1) / TY;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, ix = tile / (tiles_z * tiles_y);
-  const int y0 = ty * TY, z0 = tz * TZ;
-  // Stage the node window, entry ((xx*WY + yy)*WZ + zz)*3 + q = node
-  // (ix-1+xx, y0-1+yy, z0-1+zz) component q, and the element window, entry
-  // (ex*EY + ey)*EZ + ez = element (ix-1+ex, y0-1+ey, z0-1+ez).  Every load
-  // of a thread is issued before the first LDS write (masked loads read a
-  // valid address), so they are in flight together.
   constexpr int NUI = (NU + kBlock - 1) / kBlock, NEI = (NE + kBlock - 1) / kBlock;
   T uv[NUI], ev[NEI];
 #pragma unroll
@@ -273,6 +256,37 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
     if (t < NE) es[t] = ev[k];
   }
   __syncthreads();
+}
+
+// LDS-tiled form of the four-wave kernel.  A block covers a TY x TZ patch of
+// nodes in one x-plane (64 nodes) and first stages the u values of the
+// 3 x (TY+2) x (TZ+2) node window and the moduli of the 2 x (TY+1) x (TZ+1)
+// element window in LDS (stage_window).  Wave w then forms the element pair
+// (a,b) = (w>>1, w&1) exactly as k_fine_matvec_wave4 does, reading its window
+// from LDS, and the pair sums meet in the same fixed tree, so the result is
+// bitwise identical.  One global read per window entry replaces the
+// 36-per-lane gathers.
+template <typename T, int EPI, bool OFF, int TZ>
+__global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
+                                                             const T* __restrict__ Ke,
+                                                             const T* __restrict__ E,
+                                                             const T* __restrict__ u,
+                                                             T* __restrict__ y, EpiArgs<T> ep,
+                                                             const int* gate) {
+  using A = typename Acc<T>::type;
+  constexpr int TY = 64 / TZ;
+  constexpr int WY = TY + 2, WZ = TZ + 2;
+  constexpr int EY = TY + 1, EZ = TZ + 1;
+  constexpr int NU = 3 * WY * WZ * 3, NE = 2 * EY * EZ;
+  if (gate && *gate == 0) return;
+  __shared__ T us[NU];
+  __shared__ T es[NE];
+  __shared__ A part[3][3][64];
+  const int tiles_z = (g.NZ + TZ - 1) / TZ, tiles_y = (g.NY + TY - 1) / TY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, ix = tile / (tiles_z * tiles_y);
+  const int y0 = ty * TY, z0 = tz * TZ;
+  stage_window<T, OFF, TZ>(g, map, E, u, us, es, ix, y0, z0);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int a = wv >> 1, b = wv & 1;
@@ -314,6 +328,110 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_tile(Grid g, DofMap map,
                     map_idx<OFF>(map, 3 * n + 2)};
   const T ax[3] = {(T)r[0], (T)r[1], (T)r[2]};
   epilogue3<T, EPI>(o, ax, u, y, ep);
+}
+
+// FP64 LDS tile with the element-row contraction on the matrix cores
+// (v_mfma_f64_4x4x4f64: four independent 4x4x4 blocks per instruction).
+//
+// Same tiles, window staging, element pair per wave and summation tree as
+// k_fine_matvec_tile.  Wave w = (a,b) needs, for each of its 64 nodes and
+// c = 0, 1, the three row products t_r = sum_k Ke[3L+r][k] u_e[k] of element
+// e = (ix-1+a, iy-1+b, iz-1+c), L the node's local index in e.  One
+// instruction takes 16 nodes (4 blocks x 4 nodes) with the rows r on the M
+// axis (row 3 a zero pad) and 4 consecutive k on the K axis; six chained
+// instructions cover k = 0..23 in order.  Each FP64 4x4x4 block forms
+// D = fma(a3,b3, fma(a2,b2, fma(a1,b1, fma(a0,b0,C)))) -- the k-ordered chain
+// of fused multiply-adds, bit for bit (checked on gfx950 against an exact
+// emulation) -- so the chained instructions reproduce the VALU kernels' row
+// products exactly, and the moduli enter through the same explicit
+// P = fma(e1, t1, fma(e0, t0, 0)) as the contracted P += e*t there: all launch
+// shapes stay bitwise identical.
+// Operand lanes of the instruction: A lane 16k+4g+i holds A_g[i][k], B lane
+// 16k+4g+j holds B_g[k][j], result lane 16i+4g+j holds D_g[i][j].  Here
+// A_g[i][k] = Ke[3L+i][4s+k] (the same for every block g), B_g[k][j] =
+// u_e[4s+k] of node 16*grp+4g+j, and D_g[i][j] = t_i of that node.
+template <int EPI, bool OFF, int TZ>
+__global__ __launch_bounds__(kBlock) void k_fine_matvec_mfma(Grid g, DofMap map,
+                                                             const double* __restrict__ Ke,
+                                                             const double* __restrict__ E,
+                                                             const double* __restrict__ u,
+                                                             double* __restrict__ y,
+                                                             EpiArgs<double> ep, const int* gate) {
+  constexpr int TY = 64 / TZ;
+  constexpr int WY = TY + 2, WZ = TZ + 2;
+  constexpr int EY = TY + 1, EZ = TZ + 1;
+  constexpr int NU = 3 * WY * WZ * 3, NE = 2 * EY * EZ;
+  if (gate && *gate == 0) return;
+  __shared__ double us[NU];
+  __shared__ double es[NE];
+  __shared__ double part[3][3][64];
+  const int tiles_z = (g.NZ + TZ - 1) / TZ, tiles_y = (g.NY + TY - 1) / TY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, ix = tile / (tiles_z * tiles_y);
+  const int y0 = ty * TY, z0 = tz * TZ;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int a = wv >> 1, b = wv & 1;
+  const int hi = lane >> 4;         // k of the A and B operands, row i of the result
+  const int blk = (lane >> 2) & 3;  // 4x4x4 block
+  const int lo = lane & 3;          // row i of A; node j of B and of the result
+  // A operands (issued before the staging loads): rows 3L+lo of Ke (lo = 3
+  // pads with zeros), columns 4s+hi
+  double ak[2][6];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int L = lidx(1 - a, 1 - b, 1 - c);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const double kv = Ke[(3 * L + (lo < 3 ? lo : 0)) * 24 + 4 * s + hi];
+      ak[c][s] = lo < 3 ? kv : 0.0;
+    }
+  }
+  stage_window<double, OFF, TZ>(g, map, E, u, us, es, ix, y0, z0);
+  // B operand: element-local DOF kk = 4s+hi is component kk%3 of corner
+  // kk/3, whose window offset from the node's (ly, lz) entry is fixed per lane
+  int boff[2][6];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const int kk = 4 * s + hi, jc = kk / 3, q = kk - 3 * jc;
+      boff[c][s] = (((a + lpos(jc, 0)) * WY + (b + lpos(jc, 1))) * WZ + (c + lpos(jc, 2))) * 3 + q;
+    }
+  double P[4];
+#pragma unroll
+  for (int grp = 0; grp < 4; ++grp) {
+    const int nd = 16 * grp + 4 * blk + lo;
+    const int ly = nd / TZ, lz = nd % TZ;
+    const int nb = (ly * WZ + lz) * 3;
+    double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      d0 = __builtin_amdgcn_mfma_f64_4x4x4f64(ak[0][s], us[nb + boff[0][s]], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f64_4x4x4f64(ak[1][s], us[nb + boff[1][s]], d1, 0, 0, 0);
+    }
+    const double e0 = es[(a * EY + (ly + b)) * EZ + lz];
+    const double e1 = es[(a * EY + (ly + b)) * EZ + lz + 1];
+    P[grp] = fma(e1, d1, fma(e0, d0, 0.0));
+  }
+  if (wv > 0 && hi < 3) {
+#pragma unroll
+    for (int grp = 0; grp < 4; ++grp) part[wv - 1][hi][16 * grp + 4 * blk + lo] = P[grp];
+  }
+  __syncthreads();
+  if (wv != 0 || hi >= 3) return;
+#pragma unroll
+  for (int grp = 0; grp < 4; ++grp) {
+    const int nd = 16 * grp + 4 * blk + lo;
+    const int ly = nd / TZ, lz = nd % TZ;
+    const int iy = y0 + ly, iz = z0 + lz;
+    if ((iy < g.NY) & (iz < g.NZ)) {
+      const double r =
+          (P[grp] + part[0][hi][nd]) + (part[1][hi][nd] + part[2][hi][nd]);
+      const int o = map_idx<OFF>(map, 3 * ((ix * g.NY + iy) * g.NZ + iz) + hi);
+      if (o >= 0) epilogue1<double, EPI>(o, r, u, y, ep);
+    }
+  }
 }
 
 // Element strain energy and SIMP sensitivity (solver_v4.py:258-265):
@@ -368,7 +486,9 @@ int elem_sensitivity(const Grid& g, const DofMap& map, const double* Ke, const d
 
 static std::atomic<int> g_split_f64{4}, g_split_f32{4};
 
-static bool valid_split(int s) { return s == 1 || s == 4 || s == 8 || s == 16; }
+static bool valid_split(int s) {
+  return s == 1 || s == 4 || s == 8 || s == 16 || s == kShapeMfma8 || s == kShapeMfma16;
+}
 
 void set_fine_split(int split_f64, int split_f32) {
   if (valid_split(split_f64)) g_split_f64 = split_f64;
@@ -380,6 +500,22 @@ template <typename T, int EPI, bool OFF>
 static void launch_fine_m(int split, const Grid& g, const DofMap& map, const T* Ke, const T* E,
                           const T* u, T* y, const EpiArgs<T>& ep, const int* gate,
                           hipStream_t st) {
+  if (split == kShapeMfma8 || split == kShapeMfma16) {
+    // matrix-core contraction for FP64; the FP32 operator takes the VALU
+    // tile of the same patch
+    const int tz = split == kShapeMfma8 ? 8 : 16;
+    if constexpr (std::is_same<T, double>::value) {
+      const int tiles = g.NX * cdiv(g.NY, 64 / tz) * cdiv(g.NZ, tz);
+      if (tz == 8)
+        hipLaunchKernelGGL((k_fine_matvec_mfma<EPI, OFF, 8>), dim3(tiles), dim3(kBlock), 0, st,
+                           g, map, Ke, E, u, y, ep, gate);
+      else
+        hipLaunchKernelGGL((k_fine_matvec_mfma<EPI, OFF, 16>), dim3(tiles), dim3(kBlock), 0, st,
+                           g, map, Ke, E, u, y, ep, gate);
+      return;
+    }
+    split = tz;
+  }
   if (split == 1) {
     hipLaunchKernelGGL((k_fine_matvec_node<T, EPI, OFF>), dim3(cdiv(g.n_nodes, kBlock)),
                        dim3(kBlock), 0, st, g, map, Ke, E, u, y, ep, gate);

@@ -39,8 +39,11 @@ int hw_queues();
 enum WaitFlag { WAIT_SWEEP = 0, WAIT_PCG = 1, kWaitFlags = 2 };
 
 // Fine-operator launch shapes timed by the autotuner, in this order: 1 lane
-// per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16.
-constexpr int kFineShapes = 4;
+// per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16, and the same tiles
+// with the FP64 contraction on the matrix cores (codes 72 and 80; the FP32
+// operator runs the VALU tile of the same patch there).
+constexpr int kShapeMfma8 = 72, kShapeMfma16 = 80;
+constexpr int kFineShapes = 6;
 
 // Epilogue applied by the operator kernels to each output DOF i:
 //   PLAIN : y_i = (A x)_i

@@ -84,7 +84,10 @@ class FineOperator:
 
 
 # fine-operator launch shapes in the autotuner's timing order: (name, code)
-LAUNCH_SHAPES = (("node", 1), ("wave4", 4), ("tile8x8", 8), ("tile4x16", 16))
+# fine-operator launch shapes (gf_ctx_autotune order); the mfma shapes run the
+# FP64 element-row contraction on the matrix cores (FP32: the same VALU tile)
+LAUNCH_SHAPES = (("node", 1), ("wave4", 4), ("tile8x8", 8), ("tile4x16", 16),
+                 ("mfma8x8", 72), ("mfma4x16", 80))
 
 
 class MatrixFreeKff:
@@ -115,7 +118,8 @@ class MatrixFreeKff:
             # a fixed shape instead of timing (for profiling runs, where the
             # tracer's per-dispatch cost can reorder the shapes' timings)
             if forced not in {code for _, code in LAUNCH_SHAPES}:
-                raise ValueError(f"GFHIP_FINE_SHAPE must be one of 1, 4, 8, 16; got {forced}")
+                raise ValueError("GFHIP_FINE_SHAPE must be one of "
+                                 f"{[c for _, c in LAUNCH_SHAPES]}; got {forced}")
             _lib.call("gf_ctx_set_shape", self._ctx.h, forced)
             self.launch_shape = forced
         elif autotune:
@@ -134,7 +138,8 @@ class MatrixFreeKff:
         return self.launch_shape
 
     def set_launch_shape(self, shape: int):
-        """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles)."""
+        """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles,
+        72/80 the LDS tiles with the FP64 contraction on the matrix cores)."""
         _lib.call("gf_ctx_set_shape", self._ctx.h, int(shape))
         self.launch_shape = int(shape)
 
