@@ -103,7 +103,8 @@ def parse():
                     help="confidence gate of the gated learned form: a probability, or auto "
                          "(the smallest one whose training-set predictions are 99%% correct)")
     ap.add_argument("--labels-out", default="", help="write the training labels here")
-    ap.add_argument("--theta-out", default="", help="save the trained parameters (.npy)")
+    ap.add_argument("--theta-out", default="",
+                    help="save the trained parameters (.npz: with the training grids)")
     ap.add_argument("--batch-states", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the fine matvec loop (for rocprofv3 runs)")
@@ -486,7 +487,7 @@ def learned_form(args):
     progress("learned: pooling training inputs")
     L.attach_pooled(train, rows)
     X, feat, y_reg, y_cls = L.training_tensors(rows)
-    model = L.FloorCNN(seed=0)
+    model = L.FloorCNN(seed=0, trained_grids=sorted({tuple(r["dims"]) for r in rows}))
     progress(f"learned: training ({args.epochs} epochs)")
     torch.cuda.synchronize()
     t0 = time.perf_counter()

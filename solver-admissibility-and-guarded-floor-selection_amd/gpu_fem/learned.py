@@ -37,6 +37,7 @@ POOL = (16, 8, 8)
 # floor_labels_train.csv (both in data/ beside this module).
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_THETA = os.path.join(DATA_DIR, "floor_cnn_theta.npy")
+DEFAULT_MODEL = os.path.join(DATA_DIR, "floor_cnn.npz")
 DEFAULT_LABELS = os.path.join(DATA_DIR, "floor_labels_train.csv")
 
 
@@ -58,6 +59,17 @@ def r50_from_target(t) -> np.ndarray:
 
 def grid_features(dims) -> np.ndarray:
     return (np.log2(np.asarray(dims, dtype=np.float64)) - 5.0).astype(np.float32)
+
+
+def in_grid_range(trained_grids, dims) -> bool:
+    """Whether every axis of ``dims`` lies within the per-axis range of the
+    training grids (True when they are not recorded): the grid features
+    log2(dim) - 5 of a grid outside that range are an extrapolation."""
+    if not trained_grids:
+        return True
+    g = np.asarray(trained_grids)
+    d = np.asarray(dims)
+    return bool(np.all(d >= g.min(axis=0)) and np.all(d <= g.max(axis=0)))
 
 
 def floor_class(rho_min: float, floors=FLOORS) -> int:
@@ -108,10 +120,14 @@ def init_theta(seed: int) -> np.ndarray:
 class FloorCNN:
     """Parameters, Adam state and device scratch of the floor predictor."""
 
-    def __init__(self, theta=None, seed: int = 0):
+    def __init__(self, theta=None, seed: int = 0, trained_grids=None):
         import torch
 
         _lib.require_gpu()
+        # grids of the training states; a state on a grid outside their
+        # per-axis range is not proposed for (the policy probes it instead)
+        self.trained_grids = ([tuple(int(v) for v in g) for g in trained_grids]
+                              if trained_grids is not None else None)
         self.n_params, self._wps = _sizes()
         th = init_theta(seed) if theta is None else np.asarray(theta, dtype=np.float32)
         if th.shape != (self.n_params,):
@@ -218,13 +234,32 @@ class FloorCNN:
     def theta_host(self) -> np.ndarray:
         return self.theta.cpu().numpy().copy()
 
+    def in_range(self, dims) -> bool:
+        """Whether the network may propose for a state on ``dims``."""
+        return in_grid_range(self.trained_grids, dims)
+
     def save(self, path: str):
-        np.save(path, self.theta_host())
+        """.npz: parameters and training grids; .npy: parameters only."""
+        if path.endswith(".npz"):
+            np.savez(path, theta=self.theta_host(),
+                     trained_grids=np.asarray(self.trained_grids or [], dtype=np.int64))
+        else:
+            np.save(path, self.theta_host())
 
     @classmethod
-    def load(cls, path: str = DEFAULT_THETA):
-        """Parameters saved by ``save`` (default: the shipped trained set)."""
-        return cls(theta=np.load(path, allow_pickle=False))
+    def load(cls, path: str = None):
+        """Parameters saved by ``save`` (default: the shipped trained set).
+        A bare .npy carries no training grids; the shipped one's are those of
+        the shipped labels."""
+        path = path or (DEFAULT_MODEL if os.path.exists(DEFAULT_MODEL) else DEFAULT_THETA)
+        if path.endswith(".npz"):
+            z = np.load(path, allow_pickle=False)
+            grids = [tuple(int(v) for v in g) for g in z["trained_grids"]]
+            return cls(theta=z["theta"], trained_grids=grids or None)
+        grids = None
+        if os.path.abspath(path) == os.path.abspath(DEFAULT_THETA):
+            grids = sorted({tuple(s["dims"]) for s in read_labels(DEFAULT_LABELS)[0]})
+        return cls(theta=np.load(path, allow_pickle=False), trained_grids=grids)
 
 
 def pool_density(rho, dims):
@@ -439,28 +474,29 @@ class LearnedFloorPolicy(GuardedFloorPolicy):
         self.floors = tuple(float(f) for f in floors)
         self.confidence_threshold = float(confidence_threshold)
 
-    def _gated(self, confidence) -> bool:
-        return float(confidence) < self.confidence_threshold
+    def _gated(self, confidence, out_of_range=False) -> bool:
+        return bool(out_of_range) or float(confidence) < self.confidence_threshold
 
     def first_moduli(self, state: dict) -> np.ndarray:
         ex = state.get("extra", {})
-        if self._gated(ex.get("predicted_confidence", 1.0)):
+        if self._gated(ex.get("predicted_confidence", 1.0), ex.get("out_of_range", False)):
             return super().first_moduli(state)
         k = ex.get("predicted_class", 0)
         return self.moduli(self.floors[k], state.get("solid"), state.get("rho_field"))
 
     def run_state(self, solid=None, *, rho_field=None, seed=None, probability=None,
                   keep_x=False, predicted_class=0, predicted_r50=float("nan"),
-                  predicted_confidence=1.0):
+                  predicted_confidence=1.0, out_of_range=False):
         from .solver_v2 import dot
 
-        if self._gated(predicted_confidence):
+        if self._gated(predicted_confidence, out_of_range):
             row = super().run_state(solid, rho_field=rho_field, seed=seed,
                                     probability=probability, keep_x=keep_x)
             row.update(predicted_class=int(predicted_class),
                        predicted_rho_min=self.floors[int(predicted_class)],
                        predicted_r50=float(predicted_r50),
-                       predicted_confidence=float(predicted_confidence), gate="probe",
+                       predicted_confidence=float(predicted_confidence),
+                       gate="probe_out_of_range" if out_of_range else "probe",
                        guard_fallback=int(row["post_solve_fallback_triggered"]),
                        total_iters=(len(row["probe_history"]) - 1
                                     + sum(len(h) - 1 for _, h in row["solve_history"])))
@@ -554,7 +590,8 @@ def run_learned(model: FloorCNN, states, *, workers: int = 12, load: str = "edge
         for i in idx:
             st = _policy_state(states[i], dens[i])
             st["extra"] = {"predicted_class": int(cls[i]), "predicted_r50": float(r50[i]),
-                           "predicted_confidence": float(conf[i])}
+                           "predicted_confidence": float(conf[i]),
+                           "out_of_range": not model.in_range(dims)}
             sts.append(st)
         got = _run_chunked(session, sts, chunk, progress,
                            "learned policy " + "x".join(map(str, dims)))
@@ -579,7 +616,10 @@ def compare(plain_rows, learned_rows) -> dict:
         "proposal_agreement": sum(floor_class(p["recommended_rho_min"]) == q["predicted_class"]
                                   for p, q in zip(plain_rows, learned_rows)) / max(1, n),
         "fallback_rate": sum(q["guard_fallback"] for q in learned_rows) / max(1, n),
-        "probe_rate": sum(q.get("gate") == "probe" for q in learned_rows) / max(1, n),
+        "probe_rate": sum(q.get("gate", "").startswith("probe") for q in learned_rows)
+        / max(1, n),
+        "out_of_range_rate": sum(q.get("gate") == "probe_out_of_range" for q in learned_rows)
+        / max(1, n),
         "learned_accepted": len(acc),
         "plain_accepted": sum(int(p["solve_converged"]) for p in plain_rows),
         "max_accepted_true_rel_residual": max([q["solve_final_rel_residual"] for q in acc],

@@ -183,3 +183,22 @@ def test_trained_network_gains_over_the_plain_policy():
     assert learned_iters < plain_iters
     assert c["learned_accepted"] >= c["plain_accepted"]
     assert c["max_accepted_true_rel_residual"] <= 1e-6
+
+
+def test_out_of_range_grid_goes_through_the_probe():
+    """A state on a grid outside the shipped network's training range is not
+    proposed for: it takes the plain probe path and its row equals the plain
+    policy's (decisions and iterations)."""
+    from gpu_fem.learned import FloorCNN, label_states, run_learned, sample_states
+
+    model = FloorCNN.load()
+    assert model.trained_grids
+    dims = (24, 12, 12)
+    assert not model.in_range(dims)
+    states = sample_states(2, 31, [dims], kinds=("bernoulli",), p_range=(0.3, 0.5))
+    plain = label_states(states, workers=1)
+    learned, _ = run_learned(model, states, workers=1)
+    for p, q in zip(plain, learned):
+        assert q["gate"] == "probe_out_of_range"
+        for k in ("trigger", "recommended_rho_min", "solve_iters", "solve_converged"):
+            assert p[k] == q[k], k

@@ -275,3 +275,18 @@ def test_trace_ratio_rules():
     assert _merge_notes("", "") == ""
     assert _alignment(0.0, 1.0, 5.0) == 0.0
     assert _alignment(2.0, 3.0, 6.0) == 1.0
+
+
+def test_learned_grid_range_gate():
+    """A state whose grid lies outside the per-axis range of the training
+    grids is not proposed for (grid features would be extrapolated)."""
+    from gpu_fem.learned import DEFAULT_LABELS, in_grid_range, read_labels
+
+    grids = [(40, 20, 20), (64, 32, 16), (80, 40, 20)]
+    assert in_grid_range(grids, (80, 40, 20)) and in_grid_range(grids, (50, 30, 18))
+    assert not in_grid_range(grids, (160, 80, 40))
+    assert not in_grid_range(grids, (24, 12, 12))
+    assert not in_grid_range(grids, (81, 40, 20))
+    assert in_grid_range(None, (200, 100, 50)) and in_grid_range([], (1, 1, 1))
+    shipped = sorted({tuple(s["dims"]) for s in read_labels(DEFAULT_LABELS)[0]})
+    assert shipped and all(in_grid_range(shipped, g) for g in shipped)
