@@ -607,7 +607,7 @@ int stencil_to_dense(const Grid& g, const DofMap& map, const double* A, int n_fr
   return 0;
 }
 
-// Blocked right-looking Cholesky (lower factor, row-major, n <= 5000) over
+// Blocked right-looking Cholesky (lower factor, row-major) over
 // 64-wide panels, many work-groups per panel.  Every entry receives exactly
 // the updates of the unblocked right-looking algorithm, in the same order:
 // for pivot k ascending, D[i][j] -= D[i][k] * D[j][k] (k < j <= i), then the
@@ -802,7 +802,6 @@ int dense_add_diag(int n, double* D, double eps, hipStream_t st) {
 // all threads (row-major L for the forward sweep, its transpose Lt for the
 // backward sweep, so every row segment is contiguous).
 constexpr int kTrsvThreads = 256;
-constexpr int kTrsvMaxN = 5000;
 constexpr int kTB = 64;
 
 __global__ __launch_bounds__(kTrsvThreads) void k_chol_solve(int n, const double* __restrict__ L,

@@ -43,6 +43,10 @@ enum WaitFlag { WAIT_SWEEP = 0, WAIT_PCG = 1, kWaitFlags = 2 };
 // with the FP64 contraction on the matrix cores (codes 72 and 80; the FP32
 // operator runs the VALU tile of the same patch there).
 constexpr int kShapeMfma8 = 72, kShapeMfma16 = 80;
+
+// Largest coarsest system of the two-triangular-solve form (its right-hand
+// side is held in LDS); the explicit-inverse form has no such bound.
+constexpr int kTrsvMaxN = 5000;
 constexpr int kFineShapes = 6;
 
 // Epilogue applied by the operator kernels to each output DOF i:

@@ -1329,7 +1329,10 @@ int Gmg::setup(const double* E, const double* fine_diag, hipStream_t st) {
   if (L > 1) {
     Level& c = lv[L - 1];
     const int n = c.n_free;
-    if (n <= p.dense_chol_max && n <= 5000) {
+    // dense Cholesky when n <= dense_chol_max (multigrid_v4.py:1308); the
+    // two triangular solves keep their right-hand side in LDS (at most
+    // kTrsvMaxN entries), the explicit-inverse GEMV has no such bound
+    if (n <= p.dense_chol_max && (p.coarsest_solve == 0 || n <= kTrsvMaxN)) {
       if (ch_n != n) {
         dfree(CH);
         dfree(CHt);

@@ -671,3 +671,39 @@ def test_mgs_sweep_many_blocks_matches_per_projection_launches():
     assert i1 == i2 and c1 == c2
     assert np.all(rel <= 1e-9)
     assert np.linalg.norm(x1 - x2) <= 1e-9 * np.linalg.norm(x2)
+
+
+def test_coarsest_dense_inverse_above_5000():
+    """The explicit-inverse coarsest solve beyond the triangular form's 5,000
+    DOFs (the precision form's FP64 coarsest Cholesky at 1M elements has
+    6,825): n_levels=2 on 40x20x20 with dense_chol_max 8192 gives a
+    20x10x10 coarsest level with 7,260 free DOFs (114 panels, the last one
+    ragged), through a linear two-level cycle against the oracle's dense
+    Cholesky; the triangular form keeps the PCG there."""
+    from gpu_fem import policy as gpol
+    from gpu_fem.presets import cantilever
+
+    dims = (40, 20, 20)
+    lin = dict(n_levels=2, dense_chol_max=8192, coarse_correction_policy="none",
+               inner_krylov_steps=0, root_local_correction_mode="none")
+    s = _setup(dims, "tip", rho_min=1e-3, **lin)
+    og = s["og"]
+    og.setup(s["E"])
+    rhs = np.random.default_rng(6).standard_normal(len(s["free"]))
+    zo = og.apply(rhs)
+    st = dict(STACK)
+    st.update(lin)
+    gg = gpol._build_components(cantilever(*dims), **st)[7]
+    gg.setup(s["Et"])
+    assert gg.level_free(1).size == 7260
+    assert gg.coarse_chol_mode == "dense"
+    z = _n(gg.apply(_t(rhs)))
+    err = np.abs(z - zo).max() / np.abs(zo).max()
+    print("n=7260 inverse rel err vs oracle", err)
+    # one exact coarse solve in a linear cycle: the forms differ by
+    # ~kappa(K_c) * eps (kappa ~1e6 at rho_min 1e-3 on this level)
+    assert err <= 1e-8
+    st["coarsest_solve"] = "triangular"
+    gt = gpol._build_components(cantilever(*dims), **st)[7]
+    gt.setup(s["Et"])
+    assert gt.coarse_chol_mode == "iterative"
