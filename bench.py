@@ -37,6 +37,11 @@ FP64_PEAK_TFLOPS = 78.6
 # 1.65 MB written against 2.17 + 1.65 MB algorithmic
 # (profiles/fine_matvec_fp64_80x40x20_roofline.md)
 TRAFFIC_BYTES = 4320893
+# HBM bytes per level-1 coarse-stencil launch (80x40x20 hierarchy, rotated
+# over hierarchies beyond the Infinity Cache), from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE, scaled as above (profiles/coarse_stencil_80x40x20_roofline.md);
+# None until measured
+STENCIL_TRAFFIC_BYTES = None
 SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
                  8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
 
@@ -108,8 +113,9 @@ def parse():
     ap.add_argument("--batch-states", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the fine matvec loop (for rocprofv3 runs)")
-    ap.add_argument("--roofline-kernel", default="fp64", choices=["fp64", "bf16"],
-                    help="with --roofline-only: the FP64 operator or the BF16 MFMA form")
+    ap.add_argument("--roofline-kernel", default="fp64", choices=["fp64", "bf16", "stencil"],
+                    help="with --roofline-only: the FP64 fine operator, its BF16 MFMA form, or "
+                         "the level-1 coarse stencil rotated over set-up hierarchies")
     return ap.parse_args()
 
 
@@ -154,6 +160,156 @@ def matvec_roofline(runner, n_launch=200):
     gbs = bytes_alg / (avg_us * 1e-6) / 1e9
     tfs = flops / (avg_us * 1e-6) / 1e12
     return avg_us, bytes_alg, flops, gbs, tfs
+
+
+def stencil_roofline(session, spec, stack, n_launch=600, min_bytes=300 << 20):
+    """Average duration of the level-1 coarse operator kernel (the 27-point
+    3x3-block stencil, plain epilogue: the k_stencil_nodes family holds the
+    largest share of the policy's kernel time), launched round-robin over
+    enough set-up hierarchies that the stencils they read exceed the 256 MB
+    Infinity Cache (so every launch streams its coefficients from HBM), with
+    HIP events on the launching stream.  Algorithmic bytes per launch: the
+    27 x 9 coefficients of every free node row (1944 B), its three input
+    values (24 B, each read once) and three outputs (24 B)."""
+    import numpy as np
+    import torch
+
+    from gpu_fem import _lib
+    from gpu_fem import policy as gpol
+
+    gmgs = [r.gmg for r in session.runners]
+    n1 = int(gmgs[0]._n_free[1])
+    rows = n1 // 3
+    per = rows * 1944
+    extra = []
+    rng = np.random.default_rng(5)
+    n_e = session.runners[0].n_elem
+    while per * (len(gmgs) + len(extra)) < min_bytes:
+        comps = gpol._build_components(spec, **stack)
+        g = comps[7]
+        g.setup(torch.as_tensor(np.where(rng.random(n_e) < 0.35, 1.0, 1e-3), device="cuda"))
+        extra.append(comps)
+    hs = gmgs + [c[7] for c in extra]
+    xs = [torch.randn(n1, dtype=torch.float64, device="cuda") for _ in hs]
+    ys = [torch.empty_like(x) for x in xs]
+    lib = _lib.load()
+    st = torch.cuda.current_stream()
+    calls = [(h._h, 1, _lib.ptr(x), _lib.ptr(y), st.cuda_stream) for h, x, y in zip(hs, xs, ys)]
+    for k in range(2 * len(calls)):
+        lib.gf_gmg_level_matvec(*calls[k % len(calls)])
+    st.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(st)
+    for k in range(n_launch):
+        lib.gf_gmg_level_matvec(*calls[k % len(calls)])
+    t1.record(st)
+    st.synchronize()
+    avg_us = t0.elapsed_time(t1) * 1e3 / n_launch
+    bytes_alg = rows * (1944 + 48)
+    del extra
+    return {"avg_launch_us": avg_us, "algorithmic_bytes": bytes_alg, "rows": rows,
+            "hierarchies_rotated": len(hs), "coefficient_bytes_rotated": per * len(hs),
+            "GB_s": bytes_alg / (avg_us * 1e-6) / 1e9}
+
+
+HELDOUT_SEEDS = (41, 43, 47, 53, 59, 61, 67, 71)
+HELDOUT_P = (0.08, 0.1, 0.12, 0.15, 0.18, 0.2, 0.25, 0.3, 0.35)
+REFERENCE_RATE = 0.0282  # accepted guarded solves/s, RTX 4090, the same 72 states
+
+
+def reference_parametrisation(args):
+    """The reference's own held-out cantilever suite (80x40x20 preset, SIMP
+    p = 4.5, tip point load, seeds 41..71 x p 0.08..0.35: 72 states) as one
+    session on --workers concurrent instances, timed after the session's
+    first state; the reference took 2,550.6 s on an RTX 4090 (0.0282
+    accepted solves/s)."""
+    import numpy as np
+    import torch
+
+    from gpu_fem import policy as gpol
+
+    n = 80 * 40 * 20
+    sts = [{"solid": np.random.default_rng(s).random(n) < p, "seed": s, "probability": p}
+           for s in HELDOUT_SEEDS for p in HELDOUT_P]
+    progress(f"reference parametrisation: {len(sts)} held-out states")
+    session = gpol.PolicySession("cantilever_gpu_medium", workers=args.workers, penal=4.5,
+                                 mgs_sweep=args.mgs_sweep)
+    session.run(sts[:1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    got = session.run(sts[1:])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    acc = sum(int(r["solve_converged"]) for r in got)
+    del session
+    torch.cuda.empty_cache()
+    return {"workload": ("reference held-out suite: cantilever_gpu_medium 80x40x20, SIMP p=4.5, "
+                         "tip load, seeds 41-71 x p 0.08-0.35, one session (lambda from seed 41 "
+                         "p 0.08, untimed), states 2-72 timed"),
+            "states": len(got), "accepted": acc, "wall_s": wall, "value": acc / wall,
+            "reference_rtx4090_value": REFERENCE_RATE,
+            "ratio_vs_reference": (acc / wall) / REFERENCE_RATE,
+            "max_true_rel_residual": max(r["solve_final_rel_residual"] for r in got
+                                         if r["solve_converged"])}
+
+
+def learned_on_states(args, spec, kw, dims, warm, timed, plain_rows, wall_plain, wall_seq):
+    """The learned policy with the shipped network (gpu_fem/data) on the
+    timed states, FP32 multigrid, --workers concurrent instances: ungated
+    (every in-range state solved at the proposed floor) and gated (states
+    below the network's calibrated confidence go through the plain probe).
+    Each runs as its own session after a warm-up on the same warm-up states
+    as the plain line; the pooling and the network's forward pass are
+    inside the timed region."""
+    import numpy as np
+    import torch
+
+    from gpu_fem import learned as L
+    from gpu_fem import policy as gpol
+    from gpu_fem.batch import precision_stack
+
+    model = L.FloorCNN.load()
+    out = {"model": os.path.relpath(L.DEFAULT_MODEL if os.path.exists(L.DEFAULT_MODEL)
+                                    else L.DEFAULT_THETA, ROOT),
+           "trained_grids": [list(g) for g in (model.trained_grids or [])],
+           "in_training_range": model.in_range(dims)}
+    acc_plain = sum(int(r["solve_converged"]) for r in plain_rows)
+
+    def predict(sts):
+        X = torch.stack([L.pool_density(st["solid"].astype(np.float64), dims) for st in sts])
+        feat = torch.as_tensor(np.stack([L.grid_features(dims)] * len(sts)), device="cuda")
+        return model.predict_proba(X, feat)
+
+    for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", model.gate_for(dims))):
+        progress(f"learned ({name}, gate {gate:.3f}): warm-up and {len(timed)} timed states")
+        session = gpol.PolicySession(spec, workers=args.workers, runner_cls=L.LearnedFloorPolicy,
+                                     confidence_threshold=gate, **kw, **precision_stack("fp32"))
+
+        def attach(sts):
+            cls, r50, conf = predict(sts)
+            return [dict(st, extra={"predicted_class": int(c), "predicted_r50": float(r),
+                                    "predicted_confidence": float(q),
+                                    "out_of_range": not model.in_range(dims)})
+                    for st, c, r, q in zip(sts, cls, r50, conf)]
+
+        if warm:
+            session.run(attach(warm))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        got = session.run(attach(timed))
+        torch.cuda.synchronize()
+        w = time.perf_counter() - t0
+        c = L.compare(plain_rows, got)
+        acc = c["learned_accepted"]
+        c.update(value=acc / w, wall_s=w, confidence_gate=gate,
+                 speedup_vs_plain_fp64_one_instance=(acc / w) / (acc_plain / wall_seq),
+                 speedup_vs_plain_fp64_concurrent=(acc / w) / (acc_plain / wall_plain),
+                 mean_total_iters=float(np.mean([r["total_iters"] for r in got])))
+        out[name] = c
+        del session
+        torch.cuda.empty_cache()
+    return out
 
 
 def bf16_matvec_rate(runner, n_launch=200):
@@ -218,6 +374,26 @@ def cpu_baseline(args, dims, gpu_rows):
     it_per_state = float(np.mean([r["total_iters"] for r in acc])) if acc else float("nan")
     setups_per_state = float(np.mean([r["n_setups"] for r in acc])) if acc else float("nan")
     sec_per_state = it_per_state * per_iter + setups_per_state * t_setup
+    # one complete guarded state of the CPU-runnable configuration (cantilever
+    # 40x20x20, same parametrisation), timed end to end on the oracle: probe,
+    # classification, ladder, guard, in a fresh session (lambda estimated at
+    # its first set-up), as the reference runs a state
+    c1 = (40, 20, 20)
+    spec1 = fem.cantilever_spec(*c1, load=args.load)
+    bc1, free1, _, F1, mf1 = fem.build_components(spec1)
+    g1 = ogmg.GalerkinMatFreeGMG(mf_op=mf1, free=free1, nelx=c1[0], nely=c1[1], nelz=c1[2],
+                                 KE_UNIT=fem.KE_UNIT_3D, **stack)
+
+    def solve1(E1, mx, hist):
+        g1.setup(E1)
+        return okry.fgmres(lambda x: mf1.matvec(x, E1), F1, g1.apply, tol=1e-6, maxiter=mx,
+                           restart=300, history=hist)
+
+    mask1 = opol.bernoulli_mask(0, c1, 0.35).ravel()
+    t2 = time.perf_counter()
+    row1 = opol.run_policy(mask1, solve1, F1, lambda E1: (lambda x: mf1.matvec(x, E1)),
+                           penal=args.penal)
+    c1_s = time.perf_counter() - t2
     try:
         import threadpoolctl
 
@@ -225,6 +401,13 @@ def cpu_baseline(args, dims, gpu_rows):
         cores = max([i.get("num_threads", 1) for i in info] + [1])
     except Exception:
         cores = os.cpu_count() or 1
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                             "")
+    except OSError:
+        pass
     return {
         "value": 1.0 / sec_per_state if sec_per_state > 0 else None,
         "unit": "accepted guarded solves/s",
@@ -233,7 +416,16 @@ def cpu_baseline(args, dims, gpu_rows):
         "sample": (f"oracle (NumPy/SciPy) on state seed={seed} p={p} {dims[0]}x{dims[1]}x{dims[2]}: "
                    f"set-up {t_setup:.2f} s, one canonical FGMRES iteration {per_iter:.2f} s "
                    f"(measured), scaled by the GPU run's {it_per_state:.1f} iterations and "
-                   f"{setups_per_state:.2f} set-ups per accepted state"),
+                   f"{setups_per_state:.2f} set-ups per accepted state (the reference's count: "
+                   "probe, every ladder attempt and the selected solve from scratch)"),
+        "cpu_model": cpu_model,
+        "complete_state_40x20x20": {
+            "state": "seed 0, p 0.35, cantilever 40x20x20 (16,000 hex8), same parametrisation",
+            "seconds": c1_s, "accepted": int(row1["solve_converged"]),
+            "solves_per_s": int(row1["solve_converged"]) / c1_s,
+            "trigger": row1["trigger"], "recommended_rho_min": row1["recommended_rho_min"],
+            "solve_iters": row1["solve_iters"],
+        },
     }
 
 
@@ -637,6 +829,23 @@ def main():
         r.update(grid=list(dims), n_elem=runner.mf_op._n_elem, flops_per_launch=1152 * runner.mf_op._n_elem)
         print(json.dumps(r))
         return
+    if args.roofline_only and args.roofline_kernel == "stencil":
+        rng = np.random.default_rng(3)
+        n_e = dims[0] * dims[1] * dims[2]
+        for r in session.runners:
+            r.gmg.setup(torch.as_tensor(np.where(rng.random(n_e) < 0.35, 1.0, 1e-3),
+                                        device="cuda"))
+        r = stencil_roofline(session, spec, dict(gpol.CANONICAL_STACK, mgs_sweep=args.mgs_sweep),
+                             n_launch=3000)
+        src = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(50):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        r.update(kernel="k_stencil_nodes (level 1, plain epilogue)",
+                 copy_calibration_bytes={"read": 8 << 23, "write": 8 << 23})
+        print(json.dumps(r))
+        return
     if args.roofline_only:
         avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner, n_launch=2000)
         # calibration for HBM counters: a plain device copy of known size
@@ -669,9 +878,10 @@ def main():
 
     # warm-up: the session's first state (sets the cached lambda for every
     # instance) and then one state per instance
-    if args.warmup > 0:
+    warm_states = batch(max(args.warmup, args.workers)) if args.warmup > 0 else []
+    if warm_states:
         progress("policy: warm-up states")
-        session.run(batch(max(args.warmup, args.workers)))
+        session.run(warm_states)
     progress(f"policy: {args.steps} timed states")
     torch.cuda.synchronize()
     timed = batch(args.steps)
@@ -682,7 +892,26 @@ def main():
     accepted = sum(int(r["solve_converged"]) for r in rows)
     value = accepted / wall
 
-    avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner)
+    # the plain policy on one solver instance, same states after the same
+    # warm-up (instance 0 of the session: its rows must equal the concurrent ones)
+    progress(f"policy: the same {args.steps} states on one instance")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(session.streams[0]):
+        seq = [finish(runner.run_state(st["solid"], seed=st["seed"],
+                                       probability=st["probability"], keep_x=False))
+               for st in timed]
+        torch.cuda.current_stream().synchronize()
+    wall_seq = time.perf_counter() - t0
+    value_seq = sum(int(r["solve_converged"]) for r in seq) / wall_seq
+    seq_equal = all(a["recommended_rho_min"] == b["recommended_rho_min"]
+                    and a["solve_iters"] == b["solve_iters"]
+                    and a["solve_final_rel_residual"] == b["solve_final_rel_residual"]
+                    for a, b in zip(rows, seq))
+    learned = learned_on_states(args, spec, kw, dims, warm_states, timed, rows, wall, wall_seq)
+    refp = reference_parametrisation(args)
+    fine = matvec_roofline(runner)
+    sten = stencil_roofline(session, spec, dict(gpol.CANONICAL_STACK, mgs_sweep=args.mgs_sweep))
     bf16 = bf16_matvec_rate(runner)
 
     if args.states_csv:
@@ -724,25 +953,39 @@ def main():
             "triggers": [r["trigger"] for r in rows],
             "selected_floors": [r["recommended_rho_min"] for r in rows],
             "max_true_rel_residual": float(max(r["solve_final_rel_residual"] for r in rows)),
+            "kept_floors_continuing_the_probe": sum(int(r.get("selected_continued_probe", 0))
+                                                   for r in rows),
             "parallelism": f"single GPU, {args.workers} concurrent solver instances (one session)",
+            "plain_fp64_one_instance": {
+                "value": value_seq, "wall_s": wall_seq,
+                "rows_equal_concurrent": seq_equal,
+                "concurrent_speedup": value / value_seq},
+            "learned": learned,
+            "reference_parametrisation": refp,
         },
         "roofline": {
+            "kernel": ("k_stencil_nodes<double, plain epilogue> (27-point 3x3-block coarse "
+                       "operator, level 1 of the hierarchy; gf_gmg_level_matvec)"),
+            "bound": "hbm",
+            "achieved": sten["GB_s"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": sten["GB_s"] / HBM_PEAK_GBS,
+            "traffic": STENCIL_TRAFFIC_BYTES,
+            "note": ("the dominant kernel family of the policy (k_stencil_nodes: 30 % of the "
+                     "summed kernel time, profiles/policy_state_80x40x20_fp64_kernel_stats.md); "
+                     "0.24 flop/B, so HBM binds; timed round-robin over set-up hierarchies whose "
+                     "stencils together exceed the 256 MB Infinity Cache"),
+            **sten,
+        },
+        "fine_operator": {
             "kernel": (f"{SHAPE_KERNELS.get(runner.mf_op.launch_shape, '?')} FP64, "
                        "plain epilogue (gf_matvec_f64)"),
-            "bound": "mfma",
-            "achieved": tfs,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": tfs / FP64_PEAK_TFLOPS,
+            "bound": "fp64 vector ALU (19.3 flop/B, above the FP64 ridge of 9.8 flop/B)",
+            "achieved_tflops": fine[4], "peak_tflops": FP64_PEAK_TFLOPS,
+            "frac": fine[4] / FP64_PEAK_TFLOPS, "avg_launch_us": fine[0],
+            "algorithmic_bytes": fine[1], "flops": fine[2], "hbm_GB_s": fine[3],
             "traffic": TRAFFIC_BYTES,
-            "note": ("FP64 FMA on the vector ALUs; the kernel's 19.3 flop/B is above the FP64 "
-                     "ridge (9.8 flop/B) so compute binds; MI355X FP64 vector and matrix peaks "
-                     "are both 78.6 TFLOP/s"),
-            "avg_launch_us": avg_us,
-            "algorithmic_bytes": bytes_alg,
-            "flops": flops,
-            "hbm_GB_s": gbs,
-            "hbm_frac": gbs / HBM_PEAK_GBS,
             "launch_shape": runner.mf_op.launch_shape,
             "autotune_us": runner.mf_op.launch_timings_us,
             "bf16_mfma_form": bf16,

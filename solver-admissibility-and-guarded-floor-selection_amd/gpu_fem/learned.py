@@ -120,7 +120,7 @@ def init_theta(seed: int) -> np.ndarray:
 class FloorCNN:
     """Parameters, Adam state and device scratch of the floor predictor."""
 
-    def __init__(self, theta=None, seed: int = 0, trained_grids=None):
+    def __init__(self, theta=None, seed: int = 0, trained_grids=None, gates=None):
         import torch
 
         _lib.require_gpu()
@@ -128,6 +128,8 @@ class FloorCNN:
         # per-axis range is not proposed for (the policy probes it instead)
         self.trained_grids = ([tuple(int(v) for v in g) for g in trained_grids]
                               if trained_grids is not None else None)
+        # confidence gate per training grid: {dims: tau}
+        self.gates = {tuple(int(v) for v in k): float(t) for k, t in (gates or {}).items()}
         self.n_params, self._wps = _sizes()
         th = init_theta(seed) if theta is None else np.asarray(theta, dtype=np.float32)
         if th.shape != (self.n_params,):
@@ -238,11 +240,24 @@ class FloorCNN:
         """Whether the network may propose for a state on ``dims``."""
         return in_grid_range(self.trained_grids, dims)
 
+    def gate_for(self, dims) -> float:
+        """Confidence gate calibrated for the training grid nearest to
+        ``dims`` in element count (0 when none is recorded)."""
+        if not self.gates:
+            return 0.0
+        n = float(np.prod(dims))
+        best = min(self.gates, key=lambda g: abs(math.log(np.prod(g) / n)))
+        return self.gates[best]
+
     def save(self, path: str):
-        """.npz: parameters and training grids; .npy: parameters only."""
+        """.npz: parameters, training grids and confidence gates; .npy:
+        parameters only."""
         if path.endswith(".npz"):
+            gk = sorted(self.gates)
             np.savez(path, theta=self.theta_host(),
-                     trained_grids=np.asarray(self.trained_grids or [], dtype=np.int64))
+                     trained_grids=np.asarray(self.trained_grids or [], dtype=np.int64),
+                     gate_grids=np.asarray(gk, dtype=np.int64).reshape(-1, 3),
+                     gate_tau=np.asarray([self.gates[k] for k in gk], dtype=np.float64))
         else:
             np.save(path, self.theta_host())
 
@@ -255,7 +270,9 @@ class FloorCNN:
         if path.endswith(".npz"):
             z = np.load(path, allow_pickle=False)
             grids = [tuple(int(v) for v in g) for g in z["trained_grids"]]
-            return cls(theta=z["theta"], trained_grids=grids or None)
+            gates = {tuple(int(v) for v in g): float(t)
+                     for g, t in zip(z["gate_grids"], z["gate_tau"])} if "gate_tau" in z else {}
+            return cls(theta=z["theta"], trained_grids=grids or None, gates=gates)
         grids = None
         if os.path.abspath(path) == os.path.abspath(DEFAULT_THETA):
             grids = sorted({tuple(s["dims"]) for s in read_labels(DEFAULT_LABELS)[0]})
@@ -584,8 +601,11 @@ def run_learned(model: FloorCNN, states, *, workers: int = 12, load: str = "edge
     cls, r50, conf = predict_states(model, states, dens)
     rows = [None] * len(states)
     for dims, idx in _groups(states).items():
+        gk = dict(pk)
+        if gk.get("confidence_threshold") == "auto":
+            gk["confidence_threshold"] = model.gate_for(dims)
         session = PolicySession(cantilever(*dims, load=load), workers=min(workers, len(idx)),
-                                runner_cls=LearnedFloorPolicy, **pk)
+                                runner_cls=LearnedFloorPolicy, **gk)
         sts = []
         for i in idx:
             st = _policy_state(states[i], dens[i])
@@ -628,3 +648,89 @@ def compare(plain_rows, learned_rows) -> dict:
             (math.log10(max(q["predicted_r50"], 1e-300)) - math.log10(max(p["probe_r50"], 1e-300)))
             ** 2 for p, q in zip(plain_rows, learned_rows)]))),
     }
+
+
+def calibrate_gates(model: "FloorCNN", rows, target: float = 0.99) -> dict:
+    """Per-grid confidence gates: for each grid of ``rows`` (labelled states
+    the network was not trained on), the smallest confidence at which its
+    predictions are correct at rate ``target``."""
+    X, feat, _, y_cls = training_tensors(rows)
+    cls, _, conf = model.predict_proba(X, feat)
+    ok = cls == y_cls.cpu().numpy()
+    dims = [tuple(r["dims"]) for r in rows]
+    gates = {}
+    for g in sorted(set(dims)):
+        sel = np.array([d == g for d in dims])
+        gates[g] = confidence_for_accuracy(conf[sel], ok[sel], target)
+    return gates
+
+
+def main(argv=None):
+    """Label states with the plain policy, or train the network.
+
+    python -m gpu_fem.learned label --grids 160x80x40 --states 96 --seed 5 --out labels.csv
+    python -m gpu_fem.learned train --labels a.csv,b.csv --out floor_cnn.npz
+
+    ``train`` holds out every seventh state of each grid to calibrate the
+    per-grid confidence gates, trains on the rest (Adam, BF16 kernels) and
+    saves the parameters with the training grids and the gates."""
+    import argparse
+    import sys
+
+    import torch
+
+    ap = argparse.ArgumentParser(prog="python -m gpu_fem.learned")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    la = sub.add_parser("label")
+    la.add_argument("--grids", required=True)
+    la.add_argument("--states", type=int, required=True)
+    la.add_argument("--seed", type=int, required=True)
+    la.add_argument("--workers", type=int, default=8)
+    la.add_argument("--penal", type=float, default=3.0)
+    la.add_argument("--out", required=True)
+    tr = sub.add_parser("train")
+    tr.add_argument("--labels", required=True)
+    tr.add_argument("--epochs", type=int, default=60)
+    tr.add_argument("--target", type=float, default=0.99)
+    tr.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+
+    def say(msg):
+        print(f"[learned {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+    if a.cmd == "label":
+        grids = [tuple(int(v) for v in g.split("x")) for g in a.grids.split(",") if g]
+        states = sample_states(a.states, a.seed, grids)
+        say(f"labelling {len(states)} states on {grids}")
+        rows = label_states(states, workers=a.workers, progress=say, chunk=16, penal=a.penal)
+        write_labels(a.out, states, rows)
+        say(f"wrote {a.out}")
+        return
+    states, rows = [], []
+    for path in a.labels.split(","):
+        s_, r_ = read_labels(path)
+        states += s_
+        rows += r_
+    say(f"pooling {len(rows)} labelled states")
+    attach_pooled(states, rows)
+    by_grid = {}
+    for i, r in enumerate(rows):
+        by_grid.setdefault(tuple(r["dims"]), []).append(i)
+    hold = {i for idx in by_grid.values() for k, i in enumerate(idx) if k % 7 == 6}
+    fit_rows = [r for i, r in enumerate(rows) if i not in hold]
+    cal_rows = [r for i, r in enumerate(rows) if i in hold]
+    X, feat, y_reg, y_cls = training_tensors(fit_rows)
+    model = FloorCNN(seed=0, trained_grids=sorted(by_grid))
+    say(f"training on {len(fit_rows)} states ({a.epochs} epochs)")
+    hist = model.fit(X, feat, y_reg, y_cls, epochs=a.epochs, batch=64, lr=3e-3, seed=0)
+    torch.cuda.synchronize()
+    model.gates = calibrate_gates(model, cal_rows, a.target)
+    cls, _, _ = model.predict_proba(X, feat)
+    say(f"loss {hist[0]:.4f} -> {hist[-1]:.4f}; training accuracy "
+        f"{float(np.mean(cls == y_cls.cpu().numpy())):.3f}; gates {model.gates}")
+    model.save(a.out)
+    say(f"wrote {a.out}")
+
+
+if __name__ == "__main__":
+    main()
