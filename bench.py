@@ -108,6 +108,9 @@ def parse():
                     help="confidence gate of the gated learned form: a probability, or auto "
                          "(the smallest one whose training-set predictions are 99%% correct)")
     ap.add_argument("--labels-out", default="", help="write the training labels here")
+    ap.add_argument("--model", default="",
+                    help="learned form: evaluate this trained network (.npz with its training "
+                         "grids and per-grid confidence gates) instead of training one")
     ap.add_argument("--theta-out", default="",
                     help="save the trained parameters (.npz: with the training grids)")
     ap.add_argument("--batch-states", type=int, default=64)
@@ -270,8 +273,7 @@ def learned_on_states(args, spec, kw, dims, warm, timed, plain_rows, wall_plain,
     from gpu_fem.batch import precision_stack
 
     model = L.FloorCNN.load()
-    out = {"model": os.path.relpath(L.DEFAULT_MODEL if os.path.exists(L.DEFAULT_MODEL)
-                                    else L.DEFAULT_THETA, ROOT),
+    out = {"model": os.path.relpath(L.DEFAULT_MODEL, ROOT),
            "trained_grids": [list(g) for g in (model.trained_grids or [])],
            "in_training_range": model.in_range(dims)}
     acc_plain = sum(int(r["solve_converged"]) for r in plain_rows)
@@ -665,6 +667,8 @@ def learned_form(args):
     train = L.sample_states(args.train_states, args.train_seed, grids)
     t_label = None
     labels = args.labels or ("" if args.relabel else L.DEFAULT_LABELS)
+    if args.model:
+        return learned_eval(args, L.FloorCNN.load(args.model), pk)
     if labels and os.path.exists(labels):
         progress(f"learned: reading labels {labels}")
         train, rows = L.read_labels(labels)
@@ -781,6 +785,78 @@ def learned_form(args):
         },
     }
     print(json.dumps(line))
+
+
+def learned_eval(args, model, pk):
+    """Learned form with a trained network (--model): fresh-seed states on
+    --grid through the plain FP64 policy (one instance and concurrent
+    instances) and through the learned policy with FP32 multigrid, ungated
+    and gated by the network's per-grid calibrated confidence."""
+    import numpy as np
+    import torch
+
+    from gpu_fem import learned as L
+    from gpu_fem.batch import precision_stack
+
+    dims = tuple(int(v) for v in args.grid.split("x"))
+    kinds = tuple(k for k in args.eval_kinds.split(",") if k)
+    vf = (args.eval_volfrac, args.eval_volfrac) if args.eval_volfrac else (0.2, 0.5)
+    ev = L.sample_states(args.eval_states, args.eval_seed, [dims], kinds=kinds, volfrac_range=vf)
+    dens = [L.density_of(s) for s in ev]
+    workers = args.eval_workers or args.workers
+    if args.eval_warmup:
+        progress("learned: warm-up")
+        L.label_states(ev[:1], workers=1, load=args.load, densities=dens[:1], **pk)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rows = fn()
+        torch.cuda.synchronize()
+        return rows, time.perf_counter() - t0
+
+    def acc(rs):
+        return sum(int(r["solve_converged"]) for r in rs)
+
+    progress(f"learned: plain FP64 policy, one instance, {len(ev)} states on {dims}")
+    plain_seq, w_seq = timed(lambda: L.label_states(ev, workers=1, load=args.load,
+                                                    densities=dens, progress=progress, **pk))
+    progress(f"learned: plain FP64 policy, {workers} concurrent instances")
+    plain, w_plain = timed(lambda: L.label_states(ev, workers=workers, load=args.load,
+                                                  densities=dens, progress=progress, **pk))
+    v_seq, v_plain = acc(plain_seq) / w_seq, acc(plain) / w_plain
+    out = {}
+    for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", "auto")):
+        progress(f"learned: {name}")
+        got, wall = L.run_learned(model, ev, workers=workers, load=args.load, densities=dens,
+                                  progress=progress, confidence_threshold=gate, **pk,
+                                  **precision_stack("fp32"))
+        c = L.compare(plain, got)
+        c.update(value=acc(got) / wall, wall_s=wall,
+                 confidence_gate=model.gate_for(dims) if gate == "auto" else 0.0,
+                 speedup_vs_plain_fp64_one_instance=(acc(got) / wall) / v_seq,
+                 speedup_vs_plain_fp64_concurrent=(acc(got) / wall) / v_plain,
+                 mean_total_iters=float(np.mean([r["total_iters"] for r in got])))
+        out[name] = c
+    print(json.dumps({
+        "metric": ("accepted guarded solves/s, learned floor + FP32 multigrid + concurrent "
+                   "instances vs plain FP64"),
+        "value": out["fp32_gated"]["value"], "unit": "accepted guarded solves/s", "n_gpus": 1,
+        "steps": len(ev), "warmup": args.eval_warmup,
+        "ms_per_step": out["fp32_gated"]["wall_s"] / len(ev) * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 outer / f32 preconditioner; network bf16 (fp32 accumulation)",
+        "data": "synthetic (seeded Bernoulli and filtered-sigmoid density states)",
+        "config": {
+            "workload": (f"{len(ev)} fresh-seed states on cantilever {dims[0]}x{dims[1]}x{dims[2]} "
+                         f"({'/'.join(kinds)}), SIMP p={args.penal:g}, {args.load} load, ladder "
+                         "{1e-12,1e-3,1e-2}, restart 300, tol 1e-6"),
+            "model": args.model, "trained_grids": [list(g) for g in model.trained_grids or []],
+            "in_training_range": model.in_range(dims),
+            "plain_fp64_one_instance": {"value": v_seq, "wall_s": w_seq, "accepted": acc(plain_seq)},
+            "plain_fp64_concurrent": {"value": v_plain, "wall_s": w_plain, "accepted": acc(plain)},
+            "learned": out,
+            "parallelism": f"single GPU, {workers} concurrent solver instances"}}))
 
 
 def main():

@@ -33,12 +33,21 @@ from .presets import cantilever
 FLOORS = (1e-12, 1e-3, 1e-2)
 POOL = (16, 8, 8)
 
-# Parameters trained on the GPU by bench.py --form learned from the labels in
-# floor_labels_train.csv (both in data/ beside this module).
+# The shipped network (data/ beside this module): parameters, training grids
+# and per-grid confidence gates in floor_cnn.npz, trained on the GPU by
+#   python -m gpu_fem.learned train --labels gpu_fem/data/floor_labels_train.csv,\
+#       gpu_fem/data/floor_labels_large.csv,gpu_fem/data/floor_labels_1m.csv \
+#       --epochs 60 --out gpu_fem/data/floor_cnn.npz
+# from labels the plain policy wrote (python -m gpu_fem.learned label ...):
+# 40x20x20, 64x32x16, 80x40x20 (floor_labels_train.csv), 100x50x25,
+# 128x64x32, 160x80x40 (floor_labels_large.csv), 160x80x40 and 200x100x50
+# (floor_labels_1m.csv).
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-DEFAULT_THETA = os.path.join(DATA_DIR, "floor_cnn_theta.npy")
 DEFAULT_MODEL = os.path.join(DATA_DIR, "floor_cnn.npz")
 DEFAULT_LABELS = os.path.join(DATA_DIR, "floor_labels_train.csv")
+LABEL_FILES = [os.path.join(DATA_DIR, n) for n in ("floor_labels_train.csv",
+                                                   "floor_labels_large.csv",
+                                                   "floor_labels_1m.csv")]
 
 
 R50_LOG10_RANGE = (-12.0, 0.5)
@@ -264,19 +273,16 @@ class FloorCNN:
     @classmethod
     def load(cls, path: str = None):
         """Parameters saved by ``save`` (default: the shipped trained set).
-        A bare .npy carries no training grids; the shipped one's are those of
-        the shipped labels."""
-        path = path or (DEFAULT_MODEL if os.path.exists(DEFAULT_MODEL) else DEFAULT_THETA)
+        A bare .npy carries the parameters only (no training grids, no
+        gates)."""
+        path = path or DEFAULT_MODEL
         if path.endswith(".npz"):
             z = np.load(path, allow_pickle=False)
             grids = [tuple(int(v) for v in g) for g in z["trained_grids"]]
             gates = {tuple(int(v) for v in g): float(t)
                      for g, t in zip(z["gate_grids"], z["gate_tau"])} if "gate_tau" in z else {}
             return cls(theta=z["theta"], trained_grids=grids or None, gates=gates)
-        grids = None
-        if os.path.abspath(path) == os.path.abspath(DEFAULT_THETA):
-            grids = sorted({tuple(s["dims"]) for s in read_labels(DEFAULT_LABELS)[0]})
-        return cls(theta=np.load(path, allow_pickle=False), trained_grids=grids)
+        return cls(theta=np.load(path, allow_pickle=False))
 
 
 def pool_density(rho, dims):

@@ -166,10 +166,10 @@ def test_trained_network_gains_over_the_plain_policy():
     policy accepts is accepted."""
     import os
 
-    from gpu_fem.learned import (DEFAULT_THETA, FloorCNN, compare, label_states, run_learned,
+    from gpu_fem.learned import (DEFAULT_MODEL, FloorCNN, compare, label_states, run_learned,
                                  sample_states)
 
-    if not os.path.exists(DEFAULT_THETA):
+    if not os.path.exists(DEFAULT_MODEL):
         pytest.skip("no trained parameters shipped")
     model = FloorCNN.load()
     states = sample_states(8, 77, [(40, 20, 20)])

@@ -290,3 +290,20 @@ def test_learned_grid_range_gate():
     assert in_grid_range(None, (200, 100, 50)) and in_grid_range([], (1, 1, 1))
     shipped = sorted({tuple(s["dims"]) for s in read_labels(DEFAULT_LABELS)[0]})
     assert shipped and all(in_grid_range(shipped, g) for g in shipped)
+
+
+def test_shipped_network_file():
+    """The shipped network: parameter count of the architecture, training
+    grids spanning 16k..1M elements, a confidence gate per training grid, and
+    label files for every training grid."""
+    import numpy as np
+    from gpu_fem.learned import DEFAULT_MODEL, LABEL_FILES, read_labels
+
+    z = np.load(DEFAULT_MODEL, allow_pickle=False)
+    assert z["theta"].shape == (15588,) and z["theta"].dtype == np.float32
+    grids = {tuple(int(v) for v in g) for g in z["trained_grids"]}
+    assert (40, 20, 20) in grids and (200, 100, 50) in grids
+    assert {tuple(int(v) for v in g) for g in z["gate_grids"]} == grids
+    assert np.all((z["gate_tau"] > 0) & (z["gate_tau"] <= 1))
+    labelled = {tuple(s["dims"]) for f in LABEL_FILES for s in read_labels(f)[0]}
+    assert labelled == grids
