@@ -39,9 +39,10 @@ FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_BYTES = 4320893
 # HBM bytes per level-1 coarse-stencil launch (80x40x20 hierarchy, rotated
 # over hierarchies beyond the Infinity Cache), from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE, scaled as above (profiles/coarse_stencil_80x40x20_roofline.md);
-# None until measured
-STENCIL_TRAFFIC_BYTES = None
+# (9319.94 KiB) and WRITE_SIZE (216.78 KiB) per dispatch, scaled by the 64 MiB
+# copy's readings (x1.99929, x1.0): 1.05x the 18,406,080 algorithmic bytes
+# (profiles/coarse_stencil_80x40x20_roofline.md)
+STENCIL_TRAFFIC_BYTES = 19302430
 SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
                  8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
 
@@ -62,6 +63,9 @@ def parse():
     ap.add_argument("--probabilities", default="0.2,0.35,0.5")
     ap.add_argument("--seeds", default=",".join(str(s) for s in range(32)))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="policy line only: skip the one-instance, learned and reference-"
+                         "parametrisation runs and the roofline loops (for profiling runs)")
     ap.add_argument("--workers", type=int, default=12,
                     help="solver instances running states concurrently (one session)")
     ap.add_argument("--mgs-sweep", type=int, default=1, choices=[0, 1, 2],
@@ -968,6 +972,12 @@ def main():
     accepted = sum(int(r["solve_converged"]) for r in rows)
     value = accepted / wall
 
+    if args.no_extras:
+        print(json.dumps({"metric": "accepted guarded solves/s (policy line only)",
+                          "value": value, "unit": "accepted guarded solves/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": wall / args.steps * 1e3}))
+        return
     # the plain policy on one solver instance, same states after the same
     # warm-up (instance 0 of the session: its rows must equal the concurrent ones)
     progress(f"policy: the same {args.steps} states on one instance")
