@@ -153,18 +153,21 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
 }
 
 // Node-per-lane form of k_stencil with coalesced coefficient reads: a block
-// is 64 consecutive nodes x 4 waves; wave w evaluates the slots k = w, w+4,
-// w+8, ... for every node of the block (lane = node, so each coefficient
+// is 64 consecutive nodes x NW waves; wave w evaluates the slots k = w, w+NW,
+// w+2NW, ... for every node of the block (lane = node, so each coefficient
 // plane A[(k*9 + r*3 + q) * n_nodes + n] is read contiguously), storing the
 // slot products t_k = sum_q A[k][r][q] x[J_k, q] in LDS.  Wave 0 then folds
-// them in slot order 0..26 exactly as k_stencil does, so both forms give the
-// same bits.
-template <typename T, int EPI, bool OFF>
-__global__ __launch_bounds__(kBlock) void k_stencil_nodes(Grid g, DofMap map,
-                                                          const T* __restrict__ A,
-                                                          const T* __restrict__ x,
-                                                          T* __restrict__ y, EpiArgs<T> ep,
-                                                          const int* gate) {
+// them in slot order 0..26 exactly as k_stencil does, so every form gives the
+// same bits.  Wave 0 loads its epilogue operands (b, D^-1, x, dv of its
+// node's three DOFs) before its slot loads, so that round trip overlaps the
+// coefficient stream instead of following the fold.
+template <typename T, int EPI, bool OFF, int NW>
+__global__ __launch_bounds__(64 * NW) void k_stencil_nodes(Grid g, DofMap map,
+                                                           const T* __restrict__ A,
+                                                           const T* __restrict__ x,
+                                                           T* __restrict__ y, EpiArgs<T> ep,
+                                                           const int* gate) {
+  constexpr int R = (27 + NW - 1) / NW;  // slot rounds per wave
   if (gate && *gate == 0) return;
   __shared__ T tk[27][3][64];
   const int lane = threadIdx.x & 63;
@@ -175,13 +178,27 @@ __global__ __launch_bounds__(kBlock) void k_stencil_nodes(Grid g, DofMap map,
   int ix, iy, iz;
   node_xyz(g, nn_, ix, iy, iz);
   const long nn = g.n_nodes;
-  // wave wv owns slots wv, wv+4, ..., wv+24 (the last one for wv < 3 only,
-  // masked rather than branched over): a compile-time trip count with no
-  // branch between the rounds lets the wave's loads be issued together
+  // epilogue operands of wave 0's node (read before any slot load)
+  int o[3] = {-1, -1, -1};
+  T eb[3] = {0, 0, 0}, ed[3] = {0, 0, 0}, ex[3] = {0, 0, 0}, ev[3] = {0, 0, 0};
+  if (wv == 0) {
 #pragma unroll
-  for (int t = 0; t < 7; ++t) {
-    const bool kin = (t < 6) | (wv < 3);
-    const int k = kin ? wv + 4 * t : 0;
+    for (int r = 0; r < 3; ++r) {
+      o[r] = valid ? map_idx<OFF>(map, 3 * n + r) : -1;
+      const int i = o[r] >= 0 ? o[r] : 0;
+      eb[r] = EPI != EPI_PLAIN ? ep.b[i] : T(0);
+      ed[r] = epi_reads_x<EPI>() ? ep.dinv[i] : T(0);
+      ex[r] = epi_reads_x<EPI>() ? x[i] : T(0);
+      ev[r] = EPI == EPI_CHEB ? ep.dv[i] : T(0);
+    }
+  }
+  // wave wv owns slots wv, wv+NW, ... (the last round masked for the waves
+  // past slot 26 rather than branched over): a compile-time trip count with
+  // no branch between the rounds lets the wave's loads be issued together
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const bool kin = wv + NW * t < 27;
+    const int k = kin ? wv + NW * t : 0;
     const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
     const bool in = kin & valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) &
                     (jz >= 0) & (jz < g.NZ);
@@ -204,12 +221,11 @@ __global__ __launch_bounds__(kBlock) void k_stencil_nodes(Grid g, DofMap map,
   if (wv != 0 || !valid) return;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    const int o = map_idx<OFF>(map, 3 * n + r);
-    if (o < 0) continue;
+    if (o[r] < 0) continue;
     T acc = 0;
 #pragma unroll
     for (int kk = 0; kk < 27; ++kk) acc += tk[kk][r][lane];
-    epilogue1<T, EPI>(o, acc, x, y, ep);
+    epi_store<T, EPI>(o[r], acc, eb[r], ed[r], ev[r], ex[r], y, ep);
   }
 }
 
@@ -217,10 +233,11 @@ template <typename T, bool OFF, bool NODES>
 static void launch_stencil(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
                            const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
   const dim3 grid(NODES ? cdiv(g.n_nodes, 64) : cdiv(g.n_nodes, 8)), block(kBlock);
+  const dim3 block8(8 * 64);
 #define GF_STENCIL_LAUNCH(E)                                                                   \
   if (NODES)                                                                                   \
-    hipLaunchKernelGGL((k_stencil_nodes<T, E, OFF>), grid, block, 0, st, g, map, A, x, y, ep,  \
-                       gate);                                                                  \
+    hipLaunchKernelGGL((k_stencil_nodes<T, E, OFF, 8>), grid, block8, 0, st, g, map, A, x, y,  \
+                       ep, gate);                                                              \
   else                                                                                         \
     hipLaunchKernelGGL((k_stencil<T, E, OFF>), grid, block, 0, st, g, map, A, x, y, ep, gate);
   switch (epi) {
