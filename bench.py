@@ -86,7 +86,7 @@ def parse():
                          "coarsest: the coarsest-level PCG in two-launch vs persistent form "
                          "through preconditioner applies (use --grid 200x100x50)")
     ap.add_argument("--density-seed", type=int, default=0)
-    ap.add_argument("--floors", default="1e-12,1e-3",
+    ap.add_argument("--floors", default="1e-12,1e-2",
                     help="precision form: probe floor, then the raised ladder")
     ap.add_argument("--probe-iters", type=int, default=100)
     ap.add_argument("--restart", type=int, default=300)
@@ -318,9 +318,13 @@ def learned_on_states(args, spec, kw, dims, warm, timed, plain_rows, wall_plain,
     return out
 
 
-def bf16_matvec_rate(runner, n_launch=200):
-    """Average duration of the BF16 matrix-core fused matvec (element
-    contraction + node sum) on the full DOF layout of the same grid."""
+def fused_matvec_rates(runner, n_launch=200):
+    """Average launch duration (HIP events on the launching stream) of the
+    fused matvec's FP32 form (the FP32 owner-computes operator) and of its
+    BF16 matrix-core form (one LDS-tiled kernel, k_fine_matvec_bf16) on the
+    full DOF layout of the runner's grid.  Algorithmic HBM bytes per launch:
+    the FP32 moduli read once (4 B per element), u read and y written once
+    (2 x 4 B per DOF); useful flops 1152 per element (24 x 24 multiply-adds)."""
     import torch
 
     from gpu_fem.cuda_fused_matvec import FusedMatvec
@@ -330,22 +334,28 @@ def bf16_matvec_rate(runner, n_launch=200):
     E = torch.rand(mf._n_elem, dtype=torch.float32, device="cuda") + 1e-3
     u = torch.randn(mf._ndof, dtype=torch.float32, device="cuda")
     st = torch.cuda.current_stream()
-    for _ in range(5):
-        fm.matvec_full(u, E, dtype="bf16")
-    st.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
     u_full, y_full = fm._scratch(u.device)
     u_full.copy_(u)
-    t0.record(st)
-    for _ in range(n_launch):
-        fm._launch("bf16", u_full, y_full, E)
-    t1.record(st)
-    st.synchronize()
-    us = t0.elapsed_time(t1) * 1e3 / n_launch
+    bytes_alg = 4 * mf._n_elem + 2 * 4 * mf._ndof
     flops = 1152 * mf._n_elem
-    return {"kernel": "k_elem_bf16 + k_node_gather_f32 (gf_matvec_full_bf16)", "avg_us": us,
-            "bf16_tflops": flops / (us * 1e-6) / 1e12}
+    out = {"algorithmic_bytes": bytes_alg, "flops": flops}
+    for dt in ("fp32", "bf16"):
+        for _ in range(5):
+            fm._launch(dt, u_full, y_full, E)
+        st.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(st)
+        for _ in range(n_launch):
+            fm._launch(dt, u_full, y_full, E)
+        t1.record(st)
+        st.synchronize()
+        us = t0.elapsed_time(t1) * 1e3 / n_launch
+        out[dt] = {"avg_us": us, "hbm_GB_s": bytes_alg / (us * 1e-6) / 1e9,
+                   "tflops": flops / (us * 1e-6) / 1e12}
+    out["bf16"]["kernel"] = "k_fine_matvec_bf16 (gf_matvec_full_bf16)"
+    out["fp32"]["kernel"] = "fine operator, FP32 (gf_matvec_full_f32)"
+    return out
 
 
 def cpu_baseline(args, dims, gpu_rows):
@@ -564,7 +574,9 @@ def precision_form(args):
     the outer FGMRES, the guard residual and the compliance stay FP64).
     Reports both forms' accepted solves/s and the FP32 result's compliance and
     solution error against the FP64 result (the FP64 form is the one held to
-    the oracle), plus the fine-matvec rate at this grid."""
+    the oracle), plus the fine-matvec rate at this grid.  A third run smooths
+    the fine level with the bf16 matrix-core operator (levels 1.. FP32) and is
+    reported beside them."""
     import numpy as np
     import torch
 
@@ -585,7 +597,7 @@ def precision_form(args):
               restart=args.restart, maxiter=args.maxiter, tol=1e-6, mgs_sweep=args.mgs_sweep,
               preset_name=f"cantilever_{dims[0]}x{dims[1]}x{dims[2]}_{args.load}")
     forms, xs, roof = {}, {}, {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16"):
         progress(f"precision {prec}: build")
         runner = gpol.GuardedFloorPolicy(spec, **kw, **precision_stack(prec))
         torch.cuda.synchronize()
@@ -601,14 +613,16 @@ def precision_form(args):
             avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner)
             roof = {"avg_launch_us": avg_us, "algorithmic_bytes": bytes_alg, "flops": flops,
                     "hbm_GB_s": gbs, "fp64_tflops": tfs,
-                    "launch_shape": runner.mf_op.launch_shape}
+                    "launch_shape": runner.mf_op.launch_shape,
+                    "fused_fp32_bf16": fused_matvec_rates(runner)}
         progress(f"precision {prec}: {wall:.1f} s, floor {row['recommended_rho_min']:.0e}, "
                  f"iters {row['solve_iters']}")
         del runner
         torch.cuda.empty_cache()
-    f64, f32 = forms["fp64"], forms["fp32"]
+    f64, f32, fb = forms["fp64"], forms["fp32"], forms["bf16"]
     x64, x32 = xs["fp64"], xs["fp32"]
     sol_err = float(torch.linalg.vector_norm(x32 - x64) / torch.linalg.vector_norm(x64))
+    sol_err_b = float(torch.linalg.vector_norm(xs["bf16"] - x64) / torch.linalg.vector_norm(x64))
 
     def brief(r):
         return {k: r[k] for k in ("trigger", "recommended_rho_min", "attempted_rho_mins",
@@ -641,6 +655,17 @@ def precision_form(args):
             / abs(f64["compliance"]),
             "solution_rel_err_vs_fp64": sol_err,
             "fp32_true_rel_residual_fp64_recomputed": f32["solve_final_rel_residual"],
+            "bf16_smoother": {
+                "value": fb["solve_converged"] / fb["wall_s"],
+                "speedup_vs_plain_fp64": (fb["solve_converged"] / fb["wall_s"]) / max(v64, 1e-300),
+                "form": "fine level smoothed with the bf16 matrix-core operator, levels 1.. FP32",
+                "run": brief(fb),
+                "floor_decisions_agree": int(fb["recommended_rho_min"] == f64["recommended_rho_min"]
+                                             and fb["trigger"] == f64["trigger"]),
+                "compliance_rel_err_vs_fp64": abs(fb["compliance"] - f64["compliance"])
+                / abs(f64["compliance"]),
+                "solution_rel_err_vs_fp64": sol_err_b,
+            },
             "fine_matvec_fp64": roof,
             "parallelism": "single GPU, one solver instance",
         },
@@ -903,10 +928,16 @@ def main():
     runner = session.runners[0]
     gen = states(args)
     if args.roofline_only and args.roofline_kernel == "bf16":
-        # the BF16 matrix-core form of the fused operator (element contraction
-        # on MFMA + node gather), for its MFMA-utilisation counters
-        r = bf16_matvec_rate(runner, n_launch=2000)
-        r.update(grid=list(dims), n_elem=runner.mf_op._n_elem, flops_per_launch=1152 * runner.mf_op._n_elem)
+        # the BF16 matrix-core form of the fused operator (one LDS-tiled
+        # kernel), beside the FP32 form, for its MFMA-utilisation counters
+        r = fused_matvec_rates(runner, n_launch=2000)
+        src = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(50):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        r.update(grid=list(dims), n_elem=runner.mf_op._n_elem,
+                 copy_calibration_bytes={"read": 8 << 23, "write": 8 << 23})
         print(json.dumps(r))
         return
     if args.roofline_only and args.roofline_kernel == "stencil":
@@ -998,7 +1029,7 @@ def main():
     refp = reference_parametrisation(args)
     fine = matvec_roofline(runner)
     sten = stencil_roofline(session, spec, dict(gpol.CANONICAL_STACK, mgs_sweep=args.mgs_sweep))
-    bf16 = bf16_matvec_rate(runner)
+    bf16 = fused_matvec_rates(runner)
 
     if args.states_csv:
         import csv
@@ -1074,7 +1105,7 @@ def main():
             "traffic": TRAFFIC_BYTES,
             "launch_shape": runner.mf_op.launch_shape,
             "autotune_us": runner.mf_op.launch_timings_us,
-            "bf16_mfma_form": bf16,
+            "fused_matvec_forms": bf16,
         },
         "cpu_baseline": cpu,
     }

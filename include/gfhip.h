@@ -163,10 +163,13 @@ int32_t gf_residual_f64(gf_ctx* ctx, const double* E, const double* b, const dou
 int32_t gf_matvec_full_f32(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
                            void* stream);
 /* BF16 matrix-core form of gf_matvec_full_f32 (FusedMatvec dtype="bf16",
- * cuda_fused_matvec.py:112-246, 393-416): KE and u rounded to bf16, products
+ * cuda_fused_matvec.py:138-246, 393-416): KE and u rounded to bf16, products
  * accumulated in FP32 on v_mfma_f32_16x16x32_bf16, E applied in FP32,
- * element rows summed per node in a fixed order (deterministic).  Allocates
- * a 24-float-per-element scratch in ctx on first use. */
+ * element rows summed per node in FP32 in a fixed order (deterministic).
+ * One kernel; element results stay in LDS.  Uploads the 32x32 bf16 KE to ctx
+ * on first use.  The same operator smooths level 0 of a hierarchy created
+ * with fine_smoother = 2 ("bf16") and a reduced-precision level 0
+ * (multigrid_v4.py:1765-1780). */
 int32_t gf_matvec_full_bf16(gf_ctx* ctx, const float* E, const float* u_full, float* y_full,
                             void* stream);
 int32_t gf_diag_f64(gf_ctx* ctx, const double* E, double* diag, void* stream);

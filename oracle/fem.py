@@ -11,6 +11,11 @@ Each function cites the reference lines it restates.  Pinning (checks in
 * ``generate_bc``        fixture columns (free-DOF counts of the reference
                          configurations) + explicit small cases.
 * ``MatrixFreeKff``      assembled ``scipy.sparse`` K restricted to free DOFs.
+* ``matvec_bf16``        the assembled K built from the bf16-rounded KE
+                         applied to the bf16-rounded u (torch's own bfloat16
+                         conversion), and the exact-product identity
+                         KE_bf16 @ u_bf16 == float64 product of the rounded
+                         operands for bf16-representable inputs.
 * ``extract_diagonal``   diagonal of the assembled K_ff.
 * ``NodeBlockJacobiMF``  3x3 diagonal blocks of the assembled K, inverted.
 * ``element_energy``     energy identity sum_e E_e ce_e = u^T K(E) u against
@@ -323,6 +328,27 @@ class MatrixFreeKff:
         f_elem = (KE @ u_elem.T * E_e[None, :]).T
         y_full = np.bincount(self._edof_flat, weights=f_elem.ravel(),
                              minlength=self._ndof).astype(dt, copy=False)
+        return y_full[self._free]
+
+    def matvec_bf16(self, u_free, E_e):
+        """The fused matvec's bf16 form (cuda_fused_matvec.py:138-246,
+        343-377): u (FP32, zero on fixed DOFs) and KE (via FP32) rounded to
+        bf16 to nearest even, element products KE_bf16 @ u_e accumulated in
+        FP32, scaled by E_e in FP32, summed per DOF in FP32; y[free] as
+        float32.  The products of bf16 operands are exact in FP32; here each
+        24-term element sum and each DOF sum is formed exactly in float64
+        and rounded once to float32, a result within a few FP32 ulps of any
+        FP32 summation order (the reference's atomics have none fixed)."""
+        from .cnn import bf16_round
+
+        u_full = np.zeros(self._ndof, dtype=np.float32)
+        u_full[self._free] = np.asarray(u_free, dtype=np.float32)
+        ub = bf16_round(u_full)
+        KEb = bf16_round(self._KE.astype(np.float32))
+        f_elem = (ub[self._edof] @ KEb.T).astype(np.float32)
+        f_elem = (f_elem * np.asarray(E_e, dtype=np.float32)[:, None]).astype(np.float32)
+        y_full = np.bincount(self._edof_flat, weights=f_elem.astype(np.float64).ravel(),
+                             minlength=self._ndof).astype(np.float32)
         return y_full[self._free]
 
     def extract_diagonal(self, E_e):

@@ -523,9 +523,12 @@ class GalerkinMatFreeGMG:
         return self.apply_fine_operator(v) if lv == 0 else self._K[lv] @ v
 
     def _fine_matvec_low(self, v32):
+        # multigrid_v4.py:1765-1780
         if self._fine_smoother == "fp32":
             return self._mf.matvec(v32, self._E32)
-        raise ValueError("bf16 fine smoothing needs the fused operator")
+        if self._fine_smoother == "bf16":
+            return self._mf.matvec_bf16(v32, self._E32)
+        raise ValueError(f"Unsupported fine smoother '{self._fine_smoother}'")
 
     # -- corrections (multigrid_v4.py:867-1055) -------------------------------
     def _apply_coarse_correction(self, lv, x, residual, corr):

@@ -26,9 +26,8 @@ struct Ctx {
   double* nb_inv9 = nullptr;
   double* nb_pinv = nullptr;
   int nb_has_pinv = 0;
-  // bf16 matrix-core contraction (allocated on first use)
+  // bf16 matrix-core operator (allocated on first use)
   unsigned short* KEb = nullptr;  // 32x32 bf16 bits, zero padded
-  float* Fel = nullptr;           // 24 floats per element
   int bf16_alloc();
   DofMap map() const { return DofMap{off, d_f2f}; }
   int init(int nelx, int nely, int nelz, const double* Ke_host, const int* free_host, int nfree,
@@ -41,6 +40,11 @@ struct Ctx {
 
 // y_full = sum_e E_e KE_bf16 bf16(u_e) over all DOFs (FP32 accumulation)
 int matvec_full_bf16(Ctx& c, const float* E, const float* u_full, float* y_full, hipStream_t st);
+// The same operator on a DOF layout with an operator epilogue (the bf16 fine
+// smoother's A_op); KEb from Ctx::bf16_alloc.
+int fine_matvec_bf16(const Grid& g, const DofMap& map, const unsigned short* KEb, const float* E,
+                     const float* u, float* y, int epi, const EpiArgs<float>& ep, const int* gate,
+                     hipStream_t st);
 
 // One device-resident flexible GMRES instance (fixed maximum restart).
 struct Krylov {
@@ -98,6 +102,7 @@ struct Gmg {
   double omega_used = 0.5;
   double contrast = NAN, void_frac = NAN, E_max = NAN, E_min = NAN;
   int high = 0, fine_deg = 2, coarse_it = 2;
+  bool fine_bf16 = false;     // level 0 smooths with the bf16 matrix-core operator
   int chol_mode = 0;          // 0 none, 1 dense, 2 iterative
   int chol_regularised = 0;
   double* CH = nullptr;       // dense L (row-major)

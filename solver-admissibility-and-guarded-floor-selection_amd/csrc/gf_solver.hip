@@ -374,7 +374,6 @@ void Ctx::release() {
   dfree(Ke);
   dfree(Ke32);
   dfree(KEb);
-  dfree(Fel);
   dfree(nb_inv9);
   dfree(nb_pinv);
   work.release();
@@ -560,11 +559,13 @@ int Gmg::create(Ctx* c, const gf_gmg_params* params) {
     // the fine_smoother's operator (multigrid_v4.py:1769-1780, 1851-1864)
     const int code = level_prec_code(p, l);
     v.prec = (l == 0) ? (code == 0 ? 0 : 1) : (code == 1 ? 1 : 0);
-    if (l == 0 && v.prec == 1 && p.fine_smoother != 1) {
-      set_error(p.fine_smoother == 2
-                    ? "BF16 fine smoothing requires the fused MFMA operator (not enabled)"
-                    : "Unsupported fine smoother for a reduced-precision level 0");
+    if (l == 0 && v.prec == 1 && p.fine_smoother != 1 && p.fine_smoother != 2) {
+      set_error("Unsupported fine smoother for a reduced-precision level 0");
       return -2;
+    }
+    if (l == 0 && v.prec == 1 && p.fine_smoother == 2) {
+      GF_TRY(ctx->bf16_alloc());
+      fine_bf16 = true;
     }
     GF_TRY(dalloc(&v.diag, n));
     GF_TRY(dalloc(&v.dinv, n));
@@ -750,6 +751,8 @@ int op_T<double>(Gmg& G, int l, const double* x, double* y, int epi, const EpiAr
 template <>
 int op_T<float>(Gmg& G, int l, const float* x, float* y, int epi, const EpiArgs<float>& ep,
                 const int* gate, hipStream_t st) {
+  if (l == 0 && G.fine_bf16)
+    return fine_matvec_bf16(G.ctx->g, G.ctx->map(), G.ctx->KEb, G.E32, x, y, epi, ep, gate, st);
   if (l == 0)
     return fine_matvec<float>(G.ctx->g, G.ctx->map(), G.ctx->Ke32, G.E32, x, y, epi, ep, gate,
                               st);

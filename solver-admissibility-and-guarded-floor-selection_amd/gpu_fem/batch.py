@@ -48,14 +48,19 @@ def precision_stack(precision: str, n_levels: int = 4) -> dict:
     coarsest level solved by the FP64 Cholesky (explicit inverse) up to 8,192
     free DOFs -- at 1M elements its 6,825 DOFs, which the plain FP64 form
     (the reference's rule, dense_chol_max 5,000) solves by 80-200 fixed
-    Jacobi-PCG iterations instead.  The outer FGMRES, the guard residual and
-    the compliance stay FP64 either way."""
+    Jacobi-PCG iterations instead.  "bf16": the same, with the fine level
+    smoothed through the bf16 matrix-core operator (multigrid_v4.py:1765-1780).
+    The outer FGMRES, the guard residual and the compliance stay FP64 in
+    every form."""
     if precision == "fp64":
         return {}
     if precision == "fp32":
         return {"fine_smoother": "fp32", "level_precisions": ["fp32"] * n_levels,
                 "dense_chol_max": 8192}
-    raise ValueError("precision must be 'fp64' or 'fp32'")
+    if precision == "bf16":
+        return {"fine_smoother": "bf16", "level_precisions": ["bf16"] + ["fp32"] * (n_levels - 1),
+                "dense_chol_max": 8192}
+    raise ValueError("precision must be 'fp64', 'fp32' or 'bf16'")
 
 
 class BatchedFloorSolver:

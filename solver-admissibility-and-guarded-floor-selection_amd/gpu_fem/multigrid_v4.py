@@ -435,12 +435,22 @@ class GalerkinMatFreeGMG:
         self._is_setup = True
         self._info = self._read_info()
 
+    def _require_bf16_operator(self) -> None:
+        """BF16 smoothing of level 0 runs the fused matvec's bf16 operator
+        (multigrid_v4.py:1774-1779): like the reference, a hierarchy built
+        without ``fused_op`` refuses to smooth with it."""
+        if (self._fine_smoother == "bf16" and self.level_precisions[0] != "fp64"
+                and self._fused_op is None):
+            raise RuntimeError(
+                "BF16 fine smoothing requires enable_fused_cuda=True so the WMMA path exists.")
+
     # ----------------------------------------------------------------- apply
     def apply(self, rhs):
         import torch
 
         if not self._is_setup:
             raise RuntimeError("GalerkinMatFreeGMG.setup() must be called before apply().")
+        self._require_bf16_operator()
         r = _as_dev(rhs, torch.float64)
         out = torch.empty_like(r)
         _lib.check(self._lib.gf_gmg_apply(self._h, _lib.ptr(r), _lib.ptr(out),
@@ -451,6 +461,7 @@ class GalerkinMatFreeGMG:
         """One base cycle (V or W) without the Krylov / root-local wrappers."""
         import torch
 
+        self._require_bf16_operator()
         r = _as_dev(rhs, torch.float64)
         out = torch.empty_like(r)
         _lib.check(self._lib.gf_gmg_vcycle(self._h, _lib.ptr(r), _lib.ptr(out),
@@ -556,6 +567,7 @@ class GalerkinMatFreeGMG:
 
         if not self._is_setup:
             raise RuntimeError("GalerkinMatFreeGMG.setup() must be called before cycle_trace().")
+        self._require_bf16_operator()
         cycle = ("w" if self._params.cycle_type == 1 else "v") if cycle_type is None else cycle_type
         if cycle not in {"v", "w"}:
             raise ValueError(f"cycle_type must be 'v' or 'w'; got {cycle!r}")
@@ -719,6 +731,7 @@ class GalerkinMatFreeGMG:
 
         if not self._is_setup:
             raise RuntimeError("GalerkinMatFreeGMG.setup() must be called before apply().")
+        self._require_bf16_operator()
         bb = _as_dev(b, torch.float64)
         x = x0.clone().contiguous() if x0 is not None else torch.empty_like(bb)
         cap = int(maxiter) + int(maxiter) // max(1, int(restart)) + 2
@@ -744,6 +757,7 @@ class GalerkinMatFreeGMG:
 
         if not self._is_setup:
             raise RuntimeError("GalerkinMatFreeGMG.setup() must be called before apply().")
+        self._require_bf16_operator()
         bb = _as_dev(b, torch.float64)
         x = torch.empty_like(bb)
         cap = int(maxiter) + int(maxiter) // max(1, int(restart)) + 2

@@ -103,6 +103,12 @@ def _build_components(preset_or_spec, *, fused_op=None, **stack):
     KE = torch.as_tensor(KE_UNIT_3D, device="cuda")
     mf_op = MatrixFreeKff(edof_gpu, KE, free_gpu, len(free), bc.ndof,
                           grid=(spec.nelx, spec.nely, spec.nelz))
+    if fused_op is None and stack.get("fine_smoother") == "bf16":
+        # the bf16 smoother runs the fused operator's bf16 form, which the
+        # reference builds with enable_fused_cuda (solver_v4.py:111-119)
+        from .cuda_fused_matvec import FusedMatvec
+
+        fused_op = FusedMatvec(edof_gpu, KE, bc.ndof, grid=(spec.nelx, spec.nely, spec.nelz))
     gmg = GalerkinMatFreeGMG(mf_op=mf_op, free=free, free_gpu=free_gpu, nelx=spec.nelx,
                              nely=spec.nely, nelz=spec.nelz, KE_UNIT=KE_UNIT_3D,
                              fused_op=fused_op, **stack)

@@ -84,13 +84,20 @@ class SolverV4:
         self._matfree_op = MatrixFreeKff(self._edof_gpu, self._KE_unit_gpu, self._free_gpu,
                                          self._n_free, self.ndof, grid=self._grid_dims)
         nx, ny, nz = self._grid_dims
+        # the fused operator, for the bf16 fine smoother (solver_v4.py:111-119)
+        self._fused_op = None
+        if enable_fused_cuda:
+            from .cuda_fused_matvec import FusedMatvec
+
+            self._fused_op = FusedMatvec(self._edof_gpu, self._KE_unit_gpu, self.ndof,
+                                         grid=self._grid_dims)
         self._matfree_gmg = GalerkinMatFreeGMG(
             mf_op=self._matfree_op, free=self._free, free_gpu=self._free_gpu, nelx=nx, nely=ny,
             nelz=nz, KE_UNIT=_host(KE_UNIT), n_levels=matfree_gmg_levels,
             coarse_smooth_iters=gmg_smooth_iters, fine_smoother=gmg_fine_smoother,
             fine_smoother_degree=gmg_fine_degree, smoother_type=gmg_smoother_type,
             cheb_lower_frac=gmg_cheb_lower_frac, level_precisions=gmg_level_precisions,
-            cycle_type=gmg_cycle_type)
+            cycle_type=gmg_cycle_type, fused_op=self._fused_op)
         self._u_prev = None
         self._prev_penal: Optional[float] = None
         self.last_outer_solver = ""

@@ -73,6 +73,38 @@ def test_matvec_diag_against_assembled(dims, preset):
     assert np.abs(y32 - K @ u).max() <= 2e-5 * np.abs(K @ u).max()
 
 
+@pytest.mark.parametrize("dims", [(6, 3, 3), (5, 4, 3)])
+def test_matvec_bf16_against_assembled_bf16_K(dims):
+    """The bf16 fused form equals the assembled K built from bf16-rounded KE
+    applied to the bf16-rounded u, with the rounding done by torch's own
+    float32 -> bfloat16 conversion (an independent implementation), to FP32
+    rounding; fixed DOFs enter as zero.  It differs from the FP64 operator by
+    the operand rounding (about 2^-9 relative), which a dropped rounding step
+    would not."""
+    import torch
+
+    spec = fem.cantilever_spec(*dims)
+    bc, free, edof, F, mf = fem.build_components(spec)
+    rng = np.random.default_rng(4)
+    E = np.where(rng.random(np.prod(dims)) < 0.5, 1.0, 1e-3).astype(np.float32)
+    u = rng.standard_normal(len(free)).astype(np.float32)
+
+    def bf(a):
+        return torch.as_tensor(np.asarray(a, dtype=np.float32)).to(torch.bfloat16).to(
+            torch.float64).numpy()
+
+    u_full = np.zeros(bc.ndof)
+    u_full[free] = bf(u)
+    Kb = fem.assemble_K(edof, E.astype(np.float64), bf(fem.KE_UNIT_3D), bc.ndof)
+    ref = (Kb @ u_full)[free]
+    y = mf.matvec_bf16(u, E)
+    assert y.dtype == np.float32
+    assert np.abs(y - ref).max() <= 4e-7 * np.abs(ref).max()
+    y64 = mf.matvec(u.astype(np.float64), E.astype(np.float64))
+    gap = np.abs(y - y64).max() / np.abs(y64).max()
+    assert 1e-4 <= gap <= 2e-2
+
+
 def test_nodeblock_against_assembled_blocks():
     spec = fem._bridge((6, 3, 3), "b", (0.0, 0.25, 0.5))
     bc, free, edof, F, mf = fem.build_components(spec)
