@@ -120,6 +120,9 @@ def parse():
     ap.add_argument("--batch-states", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the fine matvec loop (for rocprofv3 runs)")
+    ap.add_argument("--fine-shapes", default="",
+                    help="with --roofline-only (fp64): comma-separated launch shapes to time "
+                         "one after the other (1, 4, 8, 16, 72, 80; default: the autotuned one)")
     ap.add_argument("--roofline-kernel", default="fp64", choices=["fp64", "bf16", "stencil"],
                     help="with --roofline-only: the FP64 fine operator, its BF16 MFMA form, or "
                          "the level-1 coarse stencil rotated over set-up hierarchies")
@@ -135,9 +138,12 @@ def states(args):
                 yield s, p
 
 
-def matvec_roofline(runner, n_launch=200):
+def matvec_roofline(runner, n_launch=200, min_bytes=300 << 20, shape=0):
     """Average duration of the FP64 fine matvec kernel on the solver's stream
-    (HIP events), and its algorithmic bytes / flops per launch."""
+    (HIP events), and its algorithmic bytes / flops per launch.  The launches
+    rotate over enough (E, u, y) buffer sets that together they exceed the
+    256 MB Infinity Cache (at least 5 sets), so each launch reads its inputs
+    from HBM.  ``shape`` selects a launch shape (0: the autotuned one)."""
     import torch
 
     from gpu_fem import _lib
@@ -145,24 +151,33 @@ def matvec_roofline(runner, n_launch=200):
     mf = runner.mf_op
     n_free = mf._n_free
     n_elem = mf._n_elem
-    E = torch.rand(n_elem, dtype=torch.float64, device="cuda") + 1e-3
-    u = torch.randn(n_free, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(u)
+    bytes_alg = 8 * n_elem + 2 * 8 * n_free
+    n_sets = max(5, -(-min_bytes // bytes_alg))
+    sets = []
+    for _ in range(n_sets):
+        E = torch.rand(n_elem, dtype=torch.float64, device="cuda") + 1e-3
+        u = torch.randn(n_free, dtype=torch.float64, device="cuda")
+        sets.append((E, u, torch.empty_like(u)))
     st = torch.cuda.current_stream()
-    args = (mf.ctx_handle, _lib.ptr(E), _lib.ptr(u), _lib.ptr(y), st.cuda_stream)
     lib = _lib.load()
-    for _ in range(10):
-        lib.gf_matvec_f64(*args)
+    prev = mf.launch_shape
+    if shape:
+        mf.set_launch_shape(shape)
+    calls = [(mf.ctx_handle, _lib.ptr(E), _lib.ptr(u), _lib.ptr(y), st.cuda_stream)
+             for E, u, y in sets]
+    for k in range(2 * n_sets):
+        lib.gf_matvec_f64(*calls[k % n_sets])
     st.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(st)
-    for _ in range(n_launch):
-        lib.gf_matvec_f64(*args)
+    for k in range(n_launch):
+        lib.gf_matvec_f64(*calls[k % n_sets])
     t1.record(st)
     st.synchronize()
+    if shape:
+        mf.set_launch_shape(prev)
     avg_us = t0.elapsed_time(t1) * 1e3 / n_launch
-    bytes_alg = 8 * n_elem + 2 * 8 * n_free
     flops = 1152 * n_elem
     gbs = bytes_alg / (avg_us * 1e-6) / 1e9
     tfs = flops / (avg_us * 1e-6) / 1e12
@@ -958,7 +973,14 @@ def main():
         print(json.dumps(r))
         return
     if args.roofline_only:
-        avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner, n_launch=2000)
+        shapes = [int(v) for v in args.fine_shapes.split(",") if v.strip()] or [0]
+        for shape in shapes:
+            avg_us, bytes_alg, flops, gbs, tfs = matvec_roofline(runner, n_launch=2000,
+                                                                 shape=shape)
+            print(json.dumps({"kernel": "gf_matvec_f64", "avg_launch_us": avg_us,
+                              "algorithmic_bytes": bytes_alg, "flops": flops, "GB_s": gbs,
+                              "fp64_tflops": tfs,
+                              "launch_shape": shape or runner.mf_op.launch_shape}))
         # calibration for HBM counters: a plain device copy of known size
         # (torch's copy kernel), 64 MiB read + 64 MiB written per launch
         src = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
@@ -966,10 +988,7 @@ def main():
         for _ in range(50):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"kernel": "gf_matvec_f64", "avg_launch_us": avg_us,
-                          "algorithmic_bytes": bytes_alg, "flops": flops, "GB_s": gbs,
-                          "fp64_tflops": tfs, "launch_shape": runner.mf_op.launch_shape,
-                          "copy_calibration_bytes": {"read": 8 << 23, "write": 8 << 23}}))
+        print(json.dumps({"copy_calibration_bytes": {"read": 8 << 23, "write": 8 << 23}}))
         return
     n_e = dims[0] * dims[1] * dims[2]
 

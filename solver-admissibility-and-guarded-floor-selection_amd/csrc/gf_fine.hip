@@ -212,9 +212,12 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_wave4(Grid g, DofMap map
 // with branch-free masked loads: us entry ((xx*WY + yy)*WZ + zz)*3 + q = node
 // (ix-1+xx, y0-1+yy, z0-1+zz) component q (0 outside the grid and on fixed
 // DOFs), es entry (ex*EY + ey)*EZ + ez = element (ix-1+ex, y0-1+ey, z0-1+ez)
-// (0 outside).  Every load of a thread is issued before the first LDS write
-// (masked loads read a valid address), so they are in flight together.
-// Ends with a barrier.
+// (0 outside).  The window is taken row by row: a (x, y) row of the node
+// window is WZ*3 consecutive DOFs in memory, so a thread keeps one position
+// cu in a row (its z and component fixed) and walks rows tu, tu + TPU, ...;
+// only the row index changes per load.  Every load of a thread is issued
+// before the first LDS write (masked loads read a valid address), so they
+// are in flight together.  Ends with a barrier.
 template <typename T, bool OFF, int TZ>
 __device__ __forceinline__ void stage_window(const Grid& g, const DofMap& map,
                                              const T* __restrict__ E, const T* __restrict__ u,
@@ -222,38 +225,48 @@ __device__ __forceinline__ void stage_window(const Grid& g, const DofMap& map,
   constexpr int TY = 64 / TZ;
   constexpr int WY = TY + 2, WZ = TZ + 2;      // node window (x: 3 planes)
   constexpr int EY = TY + 1, EZ = TZ + 1;      // element window (x: 2 planes)
-  constexpr int NU = 3 * WY * WZ * 3, NE = 2 * EY * EZ;
-  constexpr int NUI = (NU + kBlock - 1) / kBlock, NEI = (NE + kBlock - 1) / kBlock;
+  constexpr int RU = WZ * 3, NRU = 3 * WY;     // node row length, node rows
+  constexpr int TPU = kBlock / RU;             // node rows per pass
+  constexpr int NUI = (NRU + TPU - 1) / TPU;
+  constexpr int NRE = 2 * EY;                  // element rows
+  constexpr int TPE = kBlock / EZ;
+  constexpr int NEI = (NRE + TPE - 1) / TPE;
+  const int tu = threadIdx.x / RU, cu = threadIdx.x - tu * RU;
+  const int jz = z0 - 1 + cu / 3;
+  const bool zin = (tu < TPU) & ((unsigned)jz < (unsigned)g.NZ);
+  const int dz = 3 * (z0 - 1) + cu;  // DOF offset of this position in a row
+  const int te = threadIdx.x / EZ, ce = threadIdx.x - te * EZ;
+  const int cz = z0 - 1 + ce;
+  const bool ezin = (te < TPE) & ((unsigned)cz < (unsigned)g.nz);
   T uv[NUI], ev[NEI];
 #pragma unroll
   for (int k = 0; k < NUI; ++k) {
-    const int t = threadIdx.x + k * kBlock;
-    const int q = t % 3, r = t / 3;
-    const int zz = r % WZ, r2 = r / WZ, yy = r2 % WY, xx = r2 / WY;
-    const int jx = ix - 1 + xx, jy = y0 - 1 + yy, jz = z0 - 1 + zz;
-    const bool in = (t < NU) & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) & (jz >= 0) &
-                    (jz < g.NZ);
-    uv[k] = map_load<OFF>(map, u, 3 * ((jx * g.NY + jy) * g.NZ + jz) + q, in);
+    const int r = tu + k * TPU;
+    const int xx = r / WY, yy = r - xx * WY;
+    const int jx = ix - 1 + xx, jy = y0 - 1 + yy;
+    const bool in = zin & (r < NRU) & ((unsigned)jx < (unsigned)g.NX) &
+                    ((unsigned)jy < (unsigned)g.NY);
+    uv[k] = map_load<OFF>(map, u, 3 * g.NZ * (jx * g.NY + jy) + dz, in);
   }
 #pragma unroll
   for (int k = 0; k < NEI; ++k) {
-    const int t = threadIdx.x + k * kBlock;
-    const int ez = t % EZ, r = t / EZ, ey = r % EY, ex = r / EY;
-    const int cx = ix - 1 + ex, cy = y0 - 1 + ey, cz = z0 - 1 + ez;
-    const bool in = (t < NE) & (cx >= 0) & (cx < g.nx) & (cy >= 0) & (cy < g.ny) & (cz >= 0) &
-                    (cz < g.nz);
+    const int r = te + k * TPE;
+    const int ex = r / EY, ey = r - ex * EY;
+    const int cx = ix - 1 + ex, cy = y0 - 1 + ey;
+    const bool in = ezin & (r < NRE) & ((unsigned)cx < (unsigned)g.nx) &
+                    ((unsigned)cy < (unsigned)g.ny);
     const T e = E[in ? (cx * g.ny + cy) * g.nz + cz : 0];
     ev[k] = in ? e : T(0);
   }
 #pragma unroll
   for (int k = 0; k < NUI; ++k) {
-    const int t = threadIdx.x + k * kBlock;
-    if (t < NU) us[t] = uv[k];
+    const int r = tu + k * TPU;
+    if ((tu < TPU) & (r < NRU)) us[r * RU + cu] = uv[k];
   }
 #pragma unroll
   for (int k = 0; k < NEI; ++k) {
-    const int t = threadIdx.x + k * kBlock;
-    if (t < NE) es[t] = ev[k];
+    const int r = te + k * TPE;
+    if ((te < TPE) & (r < NRE)) es[r * EZ + ce] = ev[k];
   }
   __syncthreads();
 }
