@@ -11,10 +11,11 @@ spread).  Every accepted solve, in either precision, has a recomputed true
 residual <= 1e-6.
 
 Precision form at 1,000,000 elements (200x100x50 filtered-sigmoid density):
-a short guarded policy (probe 20, r50 at 10, ladder {1e-2}) with FP64 and
-with FP32 multigrid: equal decisions, and the FP32 form's compliance and
-solution within stated bounds of the FP64 form's (1e-6 and 1e-4 relative),
-with its true residual recomputed in FP64 <= 1e-6.
+a short guarded policy (probe 20, r50 at 10, ladder {1e-2}) with FP64, with
+FP32 multigrid and with the bf16 fine smoother (FP32 coarse levels): equal
+decisions, and each reduced form's compliance and solution within stated
+bounds of the FP64 form's (1e-6 and 1e-4 relative), with its true residual
+recomputed in FP64 <= 1e-6.
 """
 
 import numpy as np
@@ -93,7 +94,7 @@ def test_precision_form_1m():
     kw = dict(penal=3.0, baseline_rho_min=1e-12, raised_rho_mins=(1e-2,), probe_iters=20,
               probe_r50=10, restart=300, maxiter=300, tol=1e-6)
     rows, xs = {}, {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16"):
         runner = gpol.GuardedFloorPolicy(spec, **kw, **precision_stack(prec))
         row = runner.run_state(rho_field=rho, keep_x=True)
         torch.cuda.synchronize()
@@ -104,15 +105,17 @@ def test_precision_form_1m():
               f"compliance {row['compliance']:.12g}", flush=True)
         del runner
         torch.cuda.empty_cache()
-    a, b = rows["fp64"], rows["fp32"]
-    for k in ("trigger", "recommended_rho_min", "attempted_rho_mins", "solve_converged"):
-        assert a[k] == b[k], k
+    a = rows["fp64"]
     assert a["solve_converged"] == 1
-    assert b["solve_final_rel_residual"] <= 1e-6
-    c_err = abs(b["compliance"] - a["compliance"]) / abs(a["compliance"])
-    x_err = float(torch.linalg.vector_norm(xs["fp32"] - xs["fp64"])
-                  / torch.linalg.vector_norm(xs["fp64"]))
-    print(f"1M fp32 vs fp64: compliance rel {c_err:.2e}, solution rel {x_err:.2e}, "
-          f"iterations {b['solve_iters']} vs {a['solve_iters']}")
-    assert c_err <= 1e-6
-    assert x_err <= 1e-4
+    for prec in ("fp32", "bf16"):
+        b = rows[prec]
+        for k in ("trigger", "recommended_rho_min", "attempted_rho_mins", "solve_converged"):
+            assert a[k] == b[k], (prec, k)
+        assert b["solve_final_rel_residual"] <= 1e-6
+        c_err = abs(b["compliance"] - a["compliance"]) / abs(a["compliance"])
+        x_err = float(torch.linalg.vector_norm(xs[prec] - xs["fp64"])
+                      / torch.linalg.vector_norm(xs["fp64"]))
+        print(f"1M {prec} vs fp64: compliance rel {c_err:.2e}, solution rel {x_err:.2e}, "
+              f"iterations {b['solve_iters']} vs {a['solve_iters']}")
+        assert c_err <= 1e-6
+        assert x_err <= 1e-4

@@ -202,3 +202,36 @@ def test_out_of_range_grid_goes_through_the_probe():
         assert q["gate"] == "probe_out_of_range"
         for k in ("trigger", "recommended_rho_min", "solve_iters", "solve_converged"):
             assert p[k] == q[k], k
+
+
+def test_learned_form_at_512k():
+    """The shipped network on fresh 160x80x40 states (512,000 elements, one of
+    its training grids; seeds it never saw, Bernoulli and filtered-sigmoid):
+    ungated and with the grid's calibrated confidence gate.  Every solve the
+    learned policy accepts passes the recomputed-residual guard, it accepts
+    every state the plain policy accepts, and ungated it spends fewer FGMRES
+    iterations than the plain policy (no 100-iteration probe)."""
+    import os
+
+    from gpu_fem.learned import DEFAULT_MODEL, FloorCNN, compare, label_states, run_learned, \
+        sample_states
+
+    if not os.path.exists(DEFAULT_MODEL):
+        pytest.skip("no trained parameters shipped")
+    model = FloorCNN.load()
+    dims = (160, 80, 40)
+    assert model.in_range(dims)
+    states = sample_states(4, 4242, [dims])
+    plain = label_states(states, workers=2)
+    plain_iters = sum(len(r["probe_history"]) - 1 + sum(len(h) - 1 for _, h in r["solve_history"])
+                      for r in plain)
+    for gate in (0.0, "auto"):
+        learned, _ = run_learned(model, states, workers=2, confidence_threshold=gate)
+        c = compare(plain, learned)
+        iters = sum(r["total_iters"] for r in learned)
+        print(f"512k gate {gate}: plain iters {plain_iters}, learned iters {iters}, {c}",
+              flush=True)
+        assert c["learned_accepted"] >= c["plain_accepted"]
+        assert c["max_accepted_true_rel_residual"] <= 1e-6
+        if gate == 0.0:
+            assert iters < plain_iters
