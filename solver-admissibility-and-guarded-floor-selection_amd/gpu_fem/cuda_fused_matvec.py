@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from .pub_simp_solver import grid_from_edof
-from .solver_v2 import _Ctx, _as_dev, _host
+from .solver_v2 import _Ctx, _as_dev, _host, _tune_ctx
 
 
 class FusedMatvec:
@@ -38,6 +38,9 @@ class FusedMatvec:
         # identity DOF map: the full layout of matvec_full
         self._ctx = _Ctx(self.grid, KE_unit_gpu, np.arange(self._ndof, dtype=np.int32),
                          self._ndof)
+        # the FP32 form runs the fine operator's launch shape chosen for this
+        # grid (bitwise the same result for every shape)
+        self.launch_shape, self.launch_timings_us = _tune_ctx(self._ctx)
         self._bf16_available = True
         self._u_full = None
         self._y_full = None

@@ -71,6 +71,25 @@ class _Ctx:
             pass
 
 
+def _tune_ctx(ctx: "_Ctx"):
+    """Launch shape of a context's fine operator: GFHIP_FINE_SHAPE when set
+    (profiling runs, where the tracer's per-dispatch cost can reorder the
+    shapes' timings), else the autotuner's pick.  Returns (shape, timings
+    in microseconds per shape or None)."""
+    forced = int(os.environ.get("GFHIP_FINE_SHAPE", "0") or 0)
+    if forced:
+        if forced not in {code for _, code in LAUNCH_SHAPES}:
+            raise ValueError("GFHIP_FINE_SHAPE must be one of "
+                             f"{[c for _, c in LAUNCH_SHAPES]}; got {forced}")
+        _lib.call("gf_ctx_set_shape", ctx.h, forced)
+        return forced, None
+    us = (C.c_float * len(LAUNCH_SHAPES))()
+    shape = C.c_int32()
+    _lib.call("gf_ctx_autotune", ctx.h, C.cast(us, C.c_void_p), C.byref(shape),
+              _lib.stream_handle())
+    return int(shape.value), {name: float(us[i]) for i, (name, _) in enumerate(LAUNCH_SHAPES)}
+
+
 class FineOperator:
     """A_op = K_ff(E) bound to moduli E; a plain callable like the reference's
     ``lambda x: mf_op.matvec(x, E_e)``, recognisable by the fast solve path."""
@@ -113,28 +132,13 @@ class MatrixFreeKff:
         self._KE_diag = np.diag(self._ctx.KE).copy()
         self.launch_shape = None
         self.launch_timings_us = None
-        forced = int(os.environ.get("GFHIP_FINE_SHAPE", "0") or 0)
-        if forced:
-            # a fixed shape instead of timing (for profiling runs, where the
-            # tracer's per-dispatch cost can reorder the shapes' timings)
-            if forced not in {code for _, code in LAUNCH_SHAPES}:
-                raise ValueError("GFHIP_FINE_SHAPE must be one of "
-                                 f"{[c for _, c in LAUNCH_SHAPES]}; got {forced}")
-            _lib.call("gf_ctx_set_shape", self._ctx.h, forced)
-            self.launch_shape = forced
-        elif autotune:
-            self.autotune()
+        if autotune or os.environ.get("GFHIP_FINE_SHAPE"):
+            self.launch_shape, self.launch_timings_us = _tune_ctx(self._ctx)
 
     def autotune(self):
         """Pick the fastest fine-operator launch shape for this grid (results
         are bitwise identical for every shape)."""
-        us = (C.c_float * len(LAUNCH_SHAPES))()
-        shape = C.c_int32()
-        _lib.call("gf_ctx_autotune", self._ctx.h, C.cast(us, C.c_void_p), C.byref(shape),
-                  _lib.stream_handle())
-        self.launch_shape = int(shape.value)
-        self.launch_timings_us = {name: float(us[i])
-                                  for i, (name, _) in enumerate(LAUNCH_SHAPES)}
+        self.launch_shape, self.launch_timings_us = _tune_ctx(self._ctx)
         return self.launch_shape
 
     def set_launch_shape(self, shape: int):
