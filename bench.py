@@ -120,6 +120,10 @@ def parse():
     ap.add_argument("--batch-states", type=int, default=64)
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the fine matvec loop (for rocprofv3 runs)")
+    ap.add_argument("--coarse-form", type=int, default=1, choices=[0, 1, 16, 32, 64],
+                    help="coarse-level operator form (gf_set_coarse_form): 1 node-per-lane "
+                         "workgroups sized by level (default), 16/32/64 that form with the "
+                         "node count fixed, 0 one lane per (node, slot)")
     ap.add_argument("--fine-shapes", default="",
                     help="with --roofline-only (fp64): comma-separated launch shapes to time "
                          "one after the other (1, 4, 8, 16, 72, 80; default: the autotuned one)")
@@ -918,8 +922,11 @@ def main():
     import numpy as np
     import torch
 
+    from gpu_fem import _lib
     from gpu_fem import policy as gpol
 
+    if args.coarse_form != 1:
+        _lib.call("gf_set_coarse_form", args.coarse_form)
     if args.form == "batched":
         batched_form(args)
         return

@@ -147,9 +147,11 @@ int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* st
 /* Force a launch shape: 0 = process default, 1, 4, 8, 16, 72 or 80. */
 int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape);
 /* Process-wide launch form of the coarse-level (27-point block stencil)
- * operator kernels: 1 = one lane per node, slot groups across four waves,
- * coalesced coefficient reads (default); 0 = one lane per (node, slot).
- * Both fold the slot products in the same order: identical results. */
+ * operator kernels: 1 = node-per-lane workgroups of eight waves with
+ * coalesced coefficient reads, 64, 32 or 16 nodes per workgroup by level
+ * size (default); 64, 32 or 16 = that form with the node count fixed;
+ * 0 = one lane per (node, slot).  All fold the slot products in the same
+ * order: identical results. */
 int32_t gf_set_coarse_form(int32_t form);
 /* y_free = K_ff(E) u_free. E: n_elem moduli; u,y: n_free (device). */
 int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream);

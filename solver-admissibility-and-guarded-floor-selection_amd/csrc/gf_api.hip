@@ -128,11 +128,12 @@ int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape) {
 }
 
 int32_t gf_set_coarse_form(int32_t form) {
-  if (form != 0 && form != 1) {
-    set_error("gf_set_coarse_form: form must be 0 or 1");
+  if (form != 0 && form != 1 && form != 16 && form != 32 && form != 64) {
+    set_error("gf_set_coarse_form: form must be 0, 1, 16, 32 or 64");
     return -2;
   }
-  set_stencil_form(form);
+  set_stencil_form(form != 0);
+  set_stencil_npb(form > 1 ? form : 0);
   return 0;
 }
 

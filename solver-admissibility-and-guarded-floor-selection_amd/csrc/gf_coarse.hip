@@ -153,35 +153,40 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Grid g, DofMap map, const T*
 }
 
 // Node-per-lane form of k_stencil with coalesced coefficient reads: a block
-// is 64 consecutive nodes x NW waves; wave w evaluates the slots k = w, w+NW,
-// w+2NW, ... for every node of the block (lane = node, so each coefficient
-// plane A[(k*9 + r*3 + q) * n_nodes + n] is read contiguously), storing the
-// slot products t_k = sum_q A[k][r][q] x[J_k, q] in LDS.  Wave 0 then folds
-// them in slot order 0..26 exactly as k_stencil does, so every form gives the
-// same bits.  Wave 0 loads its epilogue operands (b, D^-1, x, dv of its
-// node's three DOFs) before its slot loads, so that round trip overlaps the
-// coefficient stream instead of following the fold.
-template <typename T, int EPI, bool OFF, int NW>
+// is NPB consecutive nodes x NW waves; lane l of wave w takes node l % NPB
+// and evaluates the slots k = w*S + l/NPB, + NW*S, ... (S = 64/NPB sub-slots
+// per wave), so each coefficient plane A[(k*9 + r*3 + q) * n_nodes + n] is
+// read as runs of NPB consecutive values, storing the slot products
+// t_k = sum_q A[k][r][q] x[J_k, q] in LDS.  Lanes l < NPB of wave 0 then fold
+// them in slot order 0..26 exactly as k_stencil does, so every form and
+// every NPB gives the same bits.  Smaller NPB spreads a small level over
+// more workgroups.  Wave 0 loads its epilogue operands (b, D^-1, x, dv of
+// its node's three DOFs) before its slot loads, so that round trip overlaps
+// the coefficient stream instead of following the fold.
+template <typename T, int EPI, bool OFF, int NW, int NPB>
 __global__ __launch_bounds__(64 * NW) void k_stencil_nodes(Grid g, DofMap map,
                                                            const T* __restrict__ A,
                                                            const T* __restrict__ x,
                                                            T* __restrict__ y, EpiArgs<T> ep,
                                                            const int* gate) {
-  constexpr int R = (27 + NW - 1) / NW;  // slot rounds per wave
+  constexpr int S = 64 / NPB;                   // sub-slots per wave and round
+  constexpr int R = (27 + NW * S - 1) / (NW * S);  // slot rounds per wave
   if (gate && *gate == 0) return;
-  __shared__ T tk[27][3][64];
+  __shared__ T tk[27][3][NPB];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n = xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
+  const int ln = lane % NPB, sub = lane / NPB;
+  const int n = xcd_tile(blockIdx.x, gridDim.x) * NPB + ln;
   const bool valid = n < g.n_nodes;
   const int nn_ = valid ? n : 0;
   int ix, iy, iz;
   node_xyz(g, nn_, ix, iy, iz);
   const long nn = g.n_nodes;
-  // epilogue operands of wave 0's node (read before any slot load)
+  const bool folder = (wv == 0) & (sub == 0);
+  // epilogue operands of the folding lane's node (read before any slot load)
   int o[3] = {-1, -1, -1};
   T eb[3] = {0, 0, 0}, ed[3] = {0, 0, 0}, ex[3] = {0, 0, 0}, ev[3] = {0, 0, 0};
-  if (wv == 0) {
+  if (folder) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       o[r] = valid ? map_idx<OFF>(map, 3 * n + r) : -1;
@@ -192,13 +197,13 @@ __global__ __launch_bounds__(64 * NW) void k_stencil_nodes(Grid g, DofMap map,
       ev[r] = EPI == EPI_CHEB ? ep.dv[i] : T(0);
     }
   }
-  // wave wv owns slots wv, wv+NW, ... (the last round masked for the waves
-  // past slot 26 rather than branched over): a compile-time trip count with
-  // no branch between the rounds lets the wave's loads be issued together
+  // rounds with a compile-time trip count (slots past 26 masked rather than
+  // branched over) let a lane's loads be issued together
 #pragma unroll
   for (int t = 0; t < R; ++t) {
-    const bool kin = wv + NW * t < 27;
-    const int k = kin ? wv + NW * t : 0;
+    const int kk = wv * S + sub + NW * S * t;
+    const bool kin = kk < 27;
+    const int k = kin ? kk : 0;
     const int jx = ix + k / 9 - 1, jy = iy + (k / 3) % 3 - 1, jz = iz + k % 3 - 1;
     const bool in = kin & valid & (jx >= 0) & (jx < g.NX) & (jy >= 0) & (jy < g.NY) &
                     (jz >= 0) & (jz < g.NZ);
@@ -212,34 +217,53 @@ __global__ __launch_bounds__(64 * NW) void k_stencil_nodes(Grid g, DofMap map,
       s2 += A[(k * 9 + 2 * 3 + q) * nn + nn_] * xv;
     }
     if (kin) {
-      tk[k][0][lane] = s0;
-      tk[k][1][lane] = s1;
-      tk[k][2][lane] = s2;
+      tk[k][0][ln] = s0;
+      tk[k][1][ln] = s1;
+      tk[k][2][ln] = s2;
     }
   }
   __syncthreads();
-  if (wv != 0 || !valid) return;
+  if (!folder || !valid) return;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     if (o[r] < 0) continue;
     T acc = 0;
 #pragma unroll
-    for (int kk = 0; kk < 27; ++kk) acc += tk[kk][r][lane];
+    for (int k = 0; k < 27; ++k) acc += tk[k][r][ln];
     epi_store<T, EPI>(o[r], acc, eb[r], ed[r], ev[r], ex[r], y, ep);
   }
+}
+
+// Nodes per workgroup of k_stencil_nodes: 64 when the level fills the
+// device with 64-node blocks, else 32 or 16 (more, smaller workgroups).
+static std::atomic<int> g_stencil_npb{0};  // 0 = by level size
+void set_stencil_npb(int npb) { g_stencil_npb = (npb == 16 || npb == 32 || npb == 64) ? npb : 0; }
+static int stencil_npb(const Grid& g) {
+  const int f = g_stencil_npb.load();
+  if (f) return f;
+  if (cdiv(g.n_nodes, 64) >= 1024) return 64;
+  if (cdiv(g.n_nodes, 32) >= 512) return 32;
+  return 16;
 }
 
 template <typename T, bool OFF, bool NODES>
 static void launch_stencil(const Grid& g, const DofMap& map, const T* A, const T* x, T* y, int epi,
                            const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
-  const dim3 grid(NODES ? cdiv(g.n_nodes, 64) : cdiv(g.n_nodes, 8)), block(kBlock);
+  const int npb = NODES ? stencil_npb(g) : 8;
+  const dim3 grid(cdiv(g.n_nodes, npb)), block(kBlock);
   const dim3 block8(8 * 64);
 #define GF_STENCIL_LAUNCH(E)                                                                   \
-  if (NODES)                                                                                   \
-    hipLaunchKernelGGL((k_stencil_nodes<T, E, OFF, 8>), grid, block8, 0, st, g, map, A, x, y,  \
-                       ep, gate);                                                              \
+  if (!NODES)                                                                                  \
+    hipLaunchKernelGGL((k_stencil<T, E, OFF>), grid, block, 0, st, g, map, A, x, y, ep, gate); \
+  else if (npb == 64)                                                                          \
+    hipLaunchKernelGGL((k_stencil_nodes<T, E, OFF, 8, 64>), grid, block8, 0, st, g, map, A, x, \
+                       y, ep, gate);                                                           \
+  else if (npb == 32)                                                                          \
+    hipLaunchKernelGGL((k_stencil_nodes<T, E, OFF, 8, 32>), grid, block8, 0, st, g, map, A, x, \
+                       y, ep, gate);                                                           \
   else                                                                                         \
-    hipLaunchKernelGGL((k_stencil<T, E, OFF>), grid, block, 0, st, g, map, A, x, y, ep, gate);
+    hipLaunchKernelGGL((k_stencil_nodes<T, E, OFF, 8, 16>), grid, block8, 0, st, g, map, A, x, \
+                       y, ep, gate);
   switch (epi) {
     case EPI_PLAIN: GF_STENCIL_LAUNCH(EPI_PLAIN) break;
     case EPI_RESID: GF_STENCIL_LAUNCH(EPI_RESID) break;

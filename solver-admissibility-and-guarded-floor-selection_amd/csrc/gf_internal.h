@@ -280,6 +280,9 @@ int stencil_matvec(const Grid& g, const DofMap& map, const T* A, const T* x, T* 
                    const EpiArgs<T>& ep, const int* gate, hipStream_t st);
 // 0 = slot-per-lane, 1 = node-per-lane coarse operator kernel (same bits)
 void set_stencil_form(int form);
+// Nodes per workgroup of the node-per-lane coarse operator (16, 32 or 64;
+// 0 = by level size).
+void set_stencil_npb(int npb);
 int stencil_diag(const Grid& g, const DofMap& map, const double* A, double* diag, hipStream_t st);
 int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
             const double* ec, double* yf, const int* gate, hipStream_t st);

@@ -576,8 +576,9 @@ def test_density_field_driver_c64(golden_dir, tmp_path):
 
 
 def test_coarse_stencil_forms_bitwise_identical():
-    """Node-per-lane and slot-per-lane coarse operator kernels give the same
-    bits on every level (all epilogues run inside a V-cycle)."""
+    """Node-per-lane (64, 32 and 16 nodes per workgroup, and the size-chosen
+    default) and slot-per-lane coarse operator kernels give the same bits on
+    every level (all epilogues run inside a V-cycle)."""
     from gpu_fem import _lib
 
     s = _setup((21, 11, 9), "bridge", rho_min=1e-3)
@@ -586,16 +587,17 @@ def test_coarse_stencil_forms_bitwise_identical():
     r = _t(np.random.default_rng(9).standard_normal(len(s["free"])))
     outs = []
     try:
-        for form in (0, 1):
+        for form in (0, 1, 64, 32, 16):
             _lib.call("gf_set_coarse_form", form)
             lv = [_n(gg.level_matvec(l, _t(np.random.default_rng(l).standard_normal(
                 len(_n(gg.level_dinv(l))))))) for l in range(1, gg._n_levels)]
             outs.append((_n(gg.vcycle(r)), lv))
     finally:
         _lib.call("gf_set_coarse_form", 1)
-    assert np.array_equal(outs[0][0], outs[1][0])
-    for a, b in zip(outs[0][1], outs[1][1]):
-        assert np.array_equal(a, b)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0])
+        for a, b in zip(outs[0][1], o[1]):
+            assert np.array_equal(a, b)
 
 
 def test_coarsest_pcg_persistent_many_blocks():
