@@ -39,10 +39,10 @@ FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_BYTES = 4320893
 # HBM bytes per level-1 coarse-stencil launch (80x40x20 hierarchy, rotated
 # over hierarchies beyond the Infinity Cache), from rocprofv3 --pmc FETCH_SIZE
-# (9319.94 KiB) and WRITE_SIZE (216.78 KiB) per dispatch, scaled by the 64 MiB
+# (9287.06 KiB) and WRITE_SIZE (216.78 KiB) per dispatch, scaled by the 64 MiB
 # copy's readings (x1.99929, x1.0): 1.05x the 18,406,080 algorithmic bytes
 # (profiles/coarse_stencil_80x40x20_roofline.md)
-STENCIL_TRAFFIC_BYTES = 19302430
+STENCIL_TRAFFIC_BYTES = 19235215
 SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
                  8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
 
@@ -1115,10 +1115,12 @@ def main():
             "unit": "GB/s",
             "frac": sten["GB_s"] / HBM_PEAK_GBS,
             "traffic": STENCIL_TRAFFIC_BYTES,
-            "note": ("the dominant kernel family of the policy (k_stencil_nodes: 30 % of the "
-                     "summed kernel time, profiles/policy_state_80x40x20_fp64_kernel_stats.md); "
-                     "0.24 flop/B, so HBM binds; timed round-robin over set-up hierarchies whose "
-                     "stencils together exceed the 256 MB Infinity Cache"),
+            "note": ("one of the policy's two largest kernel families (k_stencil_nodes 25.1 % "
+                     "of the summed kernel time, the fine operator 25.4 %, "
+                     "profiles/policy_state_80x40x20_fp64_kernel_stats.md), and the one with an "
+                     "HBM roof: 0.24 flop/B; timed round-robin over set-up hierarchies whose "
+                     "stencils together exceed the 256 MB Infinity Cache; the fine operator's "
+                     "FP64-compute roofline is reported beside it"),
             **sten,
         },
         "fine_operator": {
