@@ -124,6 +124,9 @@ def parse():
                     help="coarse-level operator form (gf_set_coarse_form): 1 node-per-lane "
                          "workgroups sized by level (default), 16/32/64 that form with the "
                          "node count fixed, 0 one lane per (node, slot)")
+    ap.add_argument("--ls-form", type=int, default=1, choices=[0, 1],
+                    help="residual line search of the coarse corrections (gf_set_ls_form): "
+                         "1 one persistent launch where it fits (default), 0 three launches")
     ap.add_argument("--fine-shapes", default="",
                     help="with --roofline-only (fp64): comma-separated launch shapes to time "
                          "one after the other (1, 4, 8, 16, 72, 80; default: the autotuned one)")
@@ -927,6 +930,8 @@ def main():
 
     if args.coarse_form != 1:
         _lib.call("gf_set_coarse_form", args.coarse_form)
+    if args.ls_form != 1:
+        _lib.call("gf_set_ls_form", args.ls_form)
     if args.form == "batched":
         batched_form(args)
         return

@@ -153,6 +153,12 @@ int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape);
  * 0 = one lane per (node, slot).  All fold the slot products in the same
  * order: identical results. */
 int32_t gf_set_coarse_form(int32_t form);
+/* Process-wide form of the residual line search of a coarse correction
+ * (multigrid_v4.py:867-931; no reference counterpart as a switch): 1 = one
+ * persistent launch for the statistics, the trial norm and the apply where
+ * its grid fits the co-residency rule (default), 0 = three launches.  Both
+ * use the same reduction trees: identical results. */
+int32_t gf_set_ls_form(int32_t form);
 /* y_free = K_ff(E) u_free. E: n_elem moduli; u,y: n_free (device). */
 int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream);
 int32_t gf_matvec_f32(gf_ctx* ctx, const float* E, const float* u, float* y, void* stream);
@@ -216,15 +222,17 @@ int32_t gf_gmg_setup(gf_gmg* g, const double* E, const double* fine_diag, void* 
 int32_t gf_gmg_info_get(gf_gmg* g, gf_gmg_info* info);
 /* out = M^{-1} rhs (the full nonlinear preconditioner).  Synchronises the
  * stream once at the end to read the wait-expiry flags: when a grid-wide
- * wait of a Gram-Schmidt sweep (inner or root-local FGMRES) or of the
- * persistent coarsest PCG gave up, the call returns -1, the error names the
- * wait, and the flags, the sweep's exchange state and the recorded graph are
- * reset (every such call, and gf_gmg_solve, reports and clears them). */
+ * wait of a Gram-Schmidt sweep (inner or root-local FGMRES), of the
+ * persistent coarsest PCG or of a one-launch line search gave up, the call
+ * returns -1, the error names the wait, and the flags, the exchange states
+ * and the recorded graph are reset (every such call, and gf_gmg_solve,
+ * reports and clears them). */
 int32_t gf_gmg_apply(gf_gmg* g, const double* rhs, double* out, void* stream);
 /* out = one base cycle (V or W) on rhs, no wrapper; wait flags as apply. */
 int32_t gf_gmg_vcycle(gf_gmg* g, const double* rhs, double* out, void* stream);
 /* No counterpart in the reference (its waits are host synchronisations that
- * cannot expire): sets wait-expiry flag `which` (0 sweep, 1 persistent PCG)
+ * cannot expire): sets wait-expiry flag `which` (0 sweep, 1 persistent PCG,
+ * 2 one-launch line search)
  * from the host, so the error path of apply / vcycle / cycle_trace / solve
  * can be exercised without any kernel actually giving up. */
 int32_t gf_gmg_set_wait_flag(gf_gmg* g, int32_t which, void* stream);

@@ -137,6 +137,15 @@ int32_t gf_set_coarse_form(int32_t form) {
   return 0;
 }
 
+int32_t gf_set_ls_form(int32_t form) {
+  if (form != 0 && form != 1) {
+    set_error("gf_set_ls_form: form must be 0 or 1");
+    return -2;
+  }
+  set_ls_form(form);
+  return 0;
+}
+
 int32_t gf_ctx_destroy(gf_ctx* ctx) {
   if (!ctx) return 0;
   ctx->c.release();
@@ -454,7 +463,8 @@ int32_t gf_gmg_cycle_trace(gf_gmg* g, const double* rhs, double* out, int32_t cy
 int32_t gf_gmg_set_wait_flag(gf_gmg* g, int32_t which, void* stream) {
   CHECK_PTR(g);
   if (which < 0 || which >= kWaitFlags) {
-    set_error("gf_gmg_set_wait_flag: which must be 0 (sweep) or 1 (persistent PCG)");
+    set_error("gf_gmg_set_wait_flag: which must be 0 (sweep), 1 (persistent PCG) or 2 (line "
+              "search)");
     return -2;
   }
   const double one = 1.0;

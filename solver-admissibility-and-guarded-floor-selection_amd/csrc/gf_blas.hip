@@ -368,6 +368,163 @@ __global__ __launch_bounds__(kBlock) void k_trial_norm(long n, const double* __r
   finish_reduce<1>(a, partials, ticket, out, 1);
 }
 
+// ---------------------------------------------------------------------------
+// Residual line search of one coarse correction (multigrid_v4.py:867-931) in
+// one persistent launch of red_grid(n) blocks.  Phase 1 forms the four
+// statistics of k_dot4_ls, phase 2 the trial residual and its norm as
+// k_trial_norm, phase 3 the accept/apply of k_ls_apply; between phases every
+// block publishes its partial sums and all blocks wait until every block
+// has arrived.  Each block then folds all partials in block order with the
+// folding block's code of finish_reduce, so every block derives the
+// statistics, the step and the trial norm bit for bit as the three launches
+// do.  Phases 2 and 3 give a thread the same indices, so y (written by both)
+// needs no ordering between blocks.
+//
+// Arrival: cnt[0] counts block arrivals over all launches; cnt[1] holds its
+// value when a launch starts (launches on one Work are ordered by their
+// stream), so round k of this launch is complete at cnt[1] + k*G.  Block 0
+// advances cnt[1] once round 2 is complete, when every block has read it.
+// The waits are bounded: an expired wait raises *err and the block goes on.
+// ---------------------------------------------------------------------------
+constexpr long kLsSpinCap = 1L << 20;
+
+__device__ __forceinline__ void ls_arrive_wait(unsigned long long* cnt, unsigned long long target,
+                                               double* err) {
+  store_ack_wait();  // this block's partials have landed
+  __hip_atomic_fetch_add(&cnt[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  long spins = 0;
+  while (__hip_atomic_load(&cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kLsSpinCap) {
+      *err = 1.0;
+      break;
+    }
+  }
+}
+
+__device__ __forceinline__ double ls_fold(const double* parts, int G, double* lds) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < G; i += blockDim.x)
+    a += __hip_atomic_load(&parts[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum(a, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ls_fused(long n, const double* __restrict__ c,
+                                                     const double* __restrict__ r,
+                                                     const double* __restrict__ Ac,
+                                                     double* __restrict__ x, double* y,
+                                                     double* stat, double max_scale, double ratio,
+                                                     double* parts, unsigned long long* cnt,
+                                                     double* err, const int* gate) {
+  if (gate && *gate == 0) return;  // uniform: written by an earlier kernel
+  __shared__ double lds[4];
+  __shared__ double sst[8];
+  __shared__ unsigned long long base_s;
+  const int G = gridDim.x, b = blockIdx.x;
+  if (threadIdx.x == 0)
+    base_s = __hip_atomic_load(&cnt[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const long st = (long)G * blockDim.x;
+  const long i0 = b * (long)blockDim.x + threadIdx.x;
+  // phase 1: c.c, r.r, Ac.Ac, r.Ac (k_dot4_ls's loop and order)
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  {
+    long i = i0;
+    for (; i + 3 * st < n; i += 4 * st) {
+      double cv[4], rv[4], av[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cv[u] = c[i + u * st];
+        rv[u] = r[i + u * st];
+        av[u] = Ac[i + u * st];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[0] += cv[u] * cv[u];
+        a[1] += rv[u] * rv[u];
+        a[2] += av[u] * av[u];
+        a[3] += rv[u] * av[u];
+      }
+    }
+    for (; i < n; i += st) {
+      double cv = c[i], rv = r[i], av = Ac[i];
+      a[0] += cv * cv;
+      a[1] += rv * rv;
+      a[2] += av * av;
+      a[3] += rv * av;
+    }
+  }
+  double bs[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) bs[p] = block_sum(a[p], lds);
+  const unsigned long long base = base_s;  // read before the first barrier of block_sum
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      __hip_atomic_store(&parts[p * kRedBlocks + b], bs[p], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    ls_arrive_wait(cnt, base + (unsigned long long)G, err);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const double t = ls_fold(parts + p * kRedBlocks, G, lds);
+    if (threadIdx.x == 0) sst[p] = t;
+  }
+  if (threadIdx.x == 0) ls_scale(sst, max_scale);
+  __syncthreads();
+  // phase 2: y = r - alpha Ac and ||y||^2 (k_trial_norm's loop and order)
+  const double s = sst[4];
+  double a2 = 0.0;
+  {
+    long i = i0;
+    for (; i + 3 * st < n; i += 4 * st) {
+      double bv[4], xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        bv[u] = r[i + u * st];
+        xv[u] = Ac[i + u * st];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double t = bv[u] - s * xv[u];
+        if (y) y[i + u * st] = t;
+        a2 += t * t;
+      }
+    }
+    for (; i < n; i += st) {
+      double t = r[i] - s * Ac[i];
+      if (y) y[i] = t;
+      a2 += t * t;
+    }
+  }
+  const double bs2 = block_sum(a2, lds);
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&parts[4 * kRedBlocks + b], bs2, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    ls_arrive_wait(cnt, base + 2ull * (unsigned long long)G, err);
+  }
+  __syncthreads();
+  {
+    const double t = ls_fold(parts + 4 * kRedBlocks, G, lds);
+    if (threadIdx.x == 0) sst[6] = sqrt(t);
+  }
+  __syncthreads();
+  if (b == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) stat[k] = sst[k];
+    __hip_atomic_store(&cnt[1], base + 2ull * (unsigned long long)G, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // phase 3: accept or reject, x += s c (k_ls_apply)
+  const double rn = sqrt(sst[1]);
+  const bool acc = (sst[5] == 0.0) && !(sst[6] > ratio * rn);
+  const double sc = acc ? sst[4] : 0.0;
+  for (long i = i0; i < n; i += st) {
+    x[i] = x[i] + sc * c[i];
+    if (y && !acc) y[i] = r[i];
+  }
+}
+
 // y = a*x + b*y  with scalars read from device (aptr/bptr may be null = 0)
 __global__ __launch_bounds__(kBlock) void k_axpby_dev(long n, const double* aptr, double asign,
                                                       const double* __restrict__ x,
@@ -540,6 +697,45 @@ int trial_norm(Work& w, long n, const double* b, const double* x, const double* 
   return 0;
 }
 
+static std::atomic<int> g_ls_form{1};
+void set_ls_form(int form) { g_ls_form = form ? 1 : 0; }
+int get_ls_form() { return g_ls_form.load(); }
+
+// The line search's blocks wait for each other like a sweep's, so its grid
+// follows the same co-residency rule (1/hw_queues() of the capacity).
+static int ls_fits(int G) {
+  static std::atomic<int> max_blocks{-1};
+  if (max_blocks.load() < 0) {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ls_fused, kBlock, 0) !=
+        hipSuccess)
+      return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    max_blocks.store(per_cu * cus);
+  }
+  return (long)hw_queues() * G <= max_blocks.load();
+}
+
+int ls_fused(Work& w, long n, const double* c, const double* r, const double* Ac, double* x,
+             double* y, double* st, double max_scale, double ratio, double* err,
+             const int* gate, hipStream_t stream) {
+  if (!get_ls_form()) return 0;
+  const int G = red_grid(n);
+  if (!ls_fits(G)) return 0;
+  hipLaunchKernelGGL(k_ls_fused, dim3(G), dim3(kBlock), 0, stream, n, c, r, Ac, x, y, st,
+                     max_scale, ratio, w.lsparts, w.lscnt, err, gate);
+  GF_CHECK_LAUNCH();
+  return 1;
+}
+
+int ls_reset(Work& w, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, 2, w.lscnt, 0ull);
+  GF_CHECK_LAUNCH();
+  return 0;
+}
+
 int axpby_dev(long n, const double* a, double asign, const double* x, const double* b, double* y,
               const int* gate, hipStream_t st) {
   hipLaunchKernelGGL(k_axpby_dev, dim3(red_grid(n)), dim3(kBlock), 0, st, n, a, asign, x, b, y,
@@ -609,6 +805,8 @@ struct Pooled {
   int* ticket;
   int* flags;
   double* ring;
+  double* lsparts;
+  unsigned long long* lscnt;
 };
 struct UncachedPool {
   std::mutex mu;
@@ -661,9 +859,12 @@ int Work::init() {
     ticket = e.ticket;
     flags = e.flags;
     ring = e.ring;
+    lsparts = e.lsparts;
+    lscnt = e.lscnt;
     sweep_G = -1;
     g_pool.free_list.pop_back();
-    return 0;
+    // the previous owner may have left the counter mid-launch (an expired wait)
+    return zero_now(lscnt, sizeof(unsigned long long) * 2);
   }
   GF_HIP(hipExtMallocWithFlags((void**)&partials, sizeof(double) * 4 * kRedBlocks,
                                hipDeviceMallocUncached));
@@ -678,6 +879,11 @@ int Work::init() {
     std::vector<unsigned long long> empty(3 * kSweepMaxBlocks, kRingEmpty);
     GF_TRY(upload_now(ring, empty.data(), sizeof(unsigned long long) * empty.size()));
   }
+  GF_HIP(hipExtMallocWithFlags((void**)&lsparts, sizeof(double) * 5 * kRedBlocks,
+                               hipDeviceMallocUncached));
+  GF_HIP(hipExtMallocWithFlags((void**)&lscnt, sizeof(unsigned long long) * 2,
+                               hipDeviceMallocUncached));
+  GF_TRY(zero_now(lscnt, sizeof(unsigned long long) * 2));
   sweep_G = -1;
   return 0;
 }
@@ -685,11 +891,14 @@ int Work::init() {
 void Work::release() {
   GF_DEVICE_LOCK();
   std::lock_guard<std::mutex> lock(g_pool.mu);
-  if (partials) g_pool.free_list.push_back(Pooled{partials, ticket, flags, ring});
+  if (partials)
+    g_pool.free_list.push_back(Pooled{partials, ticket, flags, ring, lsparts, lscnt});
   partials = nullptr;
   ticket = nullptr;
   flags = nullptr;
   ring = nullptr;
+  lsparts = nullptr;
+  lscnt = nullptr;
 }
 
 }  // namespace gf

@@ -16,6 +16,10 @@ struct Work {
   int* flags = nullptr;        // kSweepMaxBlocks ring positions of the MGS sweep
   double* ring = nullptr;      // 3 x kSweepMaxBlocks exchange slots of the MGS sweep
   int sweep_G = -1;            // grid of the last sweep launched on flags (-1: unknown)
+  // one-launch line search: 5 x kRedBlocks block partials, and an arrival
+  // counter with the value it holds when a launch starts
+  double* lsparts = nullptr;
+  unsigned long long* lscnt = nullptr;
   int init();
   void release();
 };
@@ -34,9 +38,10 @@ constexpr unsigned long long kRingEmpty = 0x7FF4DEADBEEFCAFEull;
 // at once and every wait ends.
 int hw_queues();
 
-// Slots of the wait-expiry flag (two doubles per hierarchy): a grid-wide
-// wait of a Gram-Schmidt sweep, of the persistent coarsest PCG.
-enum WaitFlag { WAIT_SWEEP = 0, WAIT_PCG = 1, kWaitFlags = 2 };
+// Slots of the wait-expiry flag (three doubles per hierarchy): a grid-wide
+// wait of a Gram-Schmidt sweep, of the persistent coarsest PCG, of the
+// one-launch line search.
+enum WaitFlag { WAIT_SWEEP = 0, WAIT_PCG = 1, WAIT_LS = 2, kWaitFlags = 3 };
 
 // Fine-operator launch shapes timed by the autotuner, in this order: 1 lane
 // per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16, and the same tiles
@@ -232,6 +237,21 @@ int mgs_sweep(Work& w, long n, const double* V, int j, int m, double* wv, double
               const int* gate, int cooperative, const GivensArgs* gv, hipStream_t st);
 int trial_norm(Work& w, long n, const double* b, const double* x, const double* s, double* y,
                double* out, const int* gate, hipStream_t st);
+// The residual line search of one coarse correction in one persistent
+// launch: the statistics of dot4_ls_step with the step (into st[0..5]), the
+// trial residual y = r - st[4] Ac (y nullable) and its norm (st[6]), and the
+// accept/apply of k_ls_apply (x += s c; y = r when rejected) -- the same
+// reduction trees and arithmetic as those three launches, so the same bits.
+// Returns 1 when launched, 0 when the grid does not fit the co-residency
+// rule (the caller then takes the three launches), < 0 on error.
+int ls_fused(Work& w, long n, const double* c, const double* r, const double* Ac, double* x,
+             double* y, double* st, double max_scale, double ratio, double* err,
+             const int* gate, hipStream_t stream);
+// Clears the one-launch line search's arrival counter (after an expired wait).
+int ls_reset(Work& w, hipStream_t stream);
+// Line-search form: 1 = one launch where it fits (default), 0 = three launches.
+void set_ls_form(int form);
+int get_ls_form();
 int axpby_dev(long n, const double* a, double asign, const double* x, const double* b, double* y,
               const int* gate, hipStream_t st);
 int scale_inv(long n, const double* x, const double* h, double thresh, double* y,

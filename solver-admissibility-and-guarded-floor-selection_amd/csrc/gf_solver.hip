@@ -708,11 +708,14 @@ int Gmg::wait_failed(const double* flags, const char* where, hipStream_t st) {
   work.sweep_G = -1;
   drop_graph();
   LAUNCH(k_fill, kWaitFlags, st, (long)kWaitFlags, d_err, 0.0, (const int*)nullptr);
+  (void)ls_reset(work, st);
   (void)hipStreamSynchronize(st);
   std::string what;
   if (flags[WAIT_SWEEP] != 0.0) what = "the Gram-Schmidt sweep";
   if (flags[WAIT_PCG] != 0.0)
     what += std::string(what.empty() ? "" : " and ") + "the persistent coarsest PCG";
+  if (flags[WAIT_LS] != 0.0)
+    what += std::string(what.empty() ? "" : " and ") + "the one-launch line search";
   set_error(std::string(where) + ": a grid-wide wait of " + what +
             " exceeded its bound; the result is not valid");
   return -1;
@@ -726,7 +729,8 @@ int Gmg::take_wait_error(const char* where, hipStream_t st) {
   GF_HIP(hipStreamSynchronize(st));
   double f[kWaitFlags];
   std::memcpy(f, hs + 8, sizeof(f));
-  if (f[WAIT_SWEEP] != 0.0 || f[WAIT_PCG] != 0.0) return wait_failed(f, where, st);
+  for (int k = 0; k < kWaitFlags; ++k)
+    if (f[k] != 0.0) return wait_failed(f, where, st);
   return 0;
 }
 
@@ -950,6 +954,13 @@ int Gmg::correct(int l, double* x, const double* residual, const double* corr,
     return 0;
   }
   GF_TRY(level_op(l, corr, v.AC, EPI_PLAIN, EpiArgs<double>{}, gate, st));
+  // the statistics, trial norm and apply in one launch where its grid fits,
+  // else in three (identical bits)
+  const int fused = ls_fused(work, n, corr, residual, v.AC, x, residual_out, v.st,
+                             p.coarse_correction_max_scale,
+                             p.coarse_correction_acceptance_ratio, d_err + WAIT_LS, gate, st);
+  if (fused < 0) return fused;
+  if (fused) return 0;
   GF_TRY(dot4_ls_step(work, n, corr, residual, v.AC, v.st, p.coarse_correction_max_scale, gate,
                       st));
   GF_TRY(trial_norm(work, n, residual, v.AC, v.st + 4, residual_out, v.st + 6, gate, st));
@@ -1517,7 +1528,9 @@ int Gmg::outer_solve(const double* b, double* x, int x0_given, double tol, int m
                             st));
       GF_HIP(hipStreamSynchronize(st));
       std::memcpy(h, hs, sizeof(double) * 2);
-      if (hs[2 + WAIT_SWEEP] != 0.0 || hs[2 + WAIT_PCG] != 0.0) {
+      bool expired = false;
+      for (int k = 0; k < kWaitFlags; ++k) expired |= hs[2 + k] != 0.0;
+      if (expired) {
         rs.valid = false;
         return wait_failed(hs + 2, "gf_gmg_solve", st);
       }

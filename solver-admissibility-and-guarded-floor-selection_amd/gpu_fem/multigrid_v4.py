@@ -775,8 +775,9 @@ class GalerkinMatFreeGMG:
         return x, int(it.value), bool(cv.value)
 
     def _set_wait_flag(self, which: int) -> None:
-        """Raise wait-expiry flag ``which`` (0 sweep, 1 persistent PCG) from
-        the host: the next apply / vcycle / cycle_trace / solve reports it."""
+        """Raise wait-expiry flag ``which`` (0 sweep, 1 persistent PCG, 2
+        one-launch line search) from the host: the next apply / vcycle /
+        cycle_trace / solve reports it."""
         _lib.check(self._lib.gf_gmg_set_wait_flag(self._h, int(which), _lib.stream_handle()),
                    "gf_gmg_set_wait_flag")
 

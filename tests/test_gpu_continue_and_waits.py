@@ -118,7 +118,8 @@ def test_wait_flags_reported_and_cleared():
     ref = r.gmg.apply(r.F)
     ref_v = r.gmg.vcycle(r.F)
     torch.cuda.synchronize()
-    for which, name in ((0, "Gram-Schmidt sweep"), (1, "persistent coarsest PCG")):
+    for which, name in ((0, "Gram-Schmidt sweep"), (1, "persistent coarsest PCG"),
+                        (2, "one-launch line search")):
         r.gmg._set_wait_flag(which)
         with pytest.raises(_lib.GfError, match=name):
             r.gmg.apply(r.F)
@@ -139,4 +140,4 @@ def test_wait_flags_reported_and_cleared():
     x2, it2, _ = r.gmg.solve(r.F, tol=1e-6, maxiter=30, restart=300, history=h2)
     assert it1 == it2 and h1 == h2 and torch.equal(x1, x2)
     with pytest.raises(ValueError):
-        r.gmg._set_wait_flag(2)
+        r.gmg._set_wait_flag(3)

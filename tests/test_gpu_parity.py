@@ -600,6 +600,47 @@ def test_coarse_stencil_forms_bitwise_identical():
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("dims", [(16, 8, 8), (80, 40, 20)])
+def test_line_search_one_launch_bitwise(dims):
+    """The residual line search of every coarse correction in one persistent
+    launch (statistics, step, trial norm and apply) gives the bits of the
+    three-launch form: whole preconditioner applications (every level's
+    corrections, eager launches and graph replays) and an FGMRES solve with
+    its history.  Parity of that form against the oracle is
+    test_preconditioner_parity's."""
+    import torch
+    from gpu_fem import _lib, policy as gpol
+    from gpu_fem.presets import cantilever
+
+    mask = np.random.default_rng(3).random(int(np.prod(dims))) < 0.35
+    E = _t(np.where(mask, 1.0, 1e-3))
+    outs = []
+    try:
+        for form in (0, 1):
+            _lib.call("gf_set_ls_form", form)
+            res = []
+            for graph in (False, True):
+                comps = gpol._build_components(cantilever(*dims), **dict(STACK, graph_apply=graph))
+                F, gg = comps[5], comps[7]
+                gg.setup(E)
+                r = _t(np.random.default_rng(5).standard_normal(len(comps[2])))
+                res.append(_n(gg.apply(r)))
+                res.append(_n(gg.apply(F)))
+                h: list = []
+                x, it, _ = gg.solve(F, tol=1e-6, maxiter=40, restart=300, history=h)
+                res.extend([_n(x), it, h])
+                del comps, gg
+            outs.append(res)
+    finally:
+        _lib.call("gf_set_ls_form", 1)
+    torch.cuda.synchronize()
+    for a, b in zip(outs[0], outs[1]):
+        if isinstance(a, np.ndarray):
+            assert np.array_equal(a, b)
+        else:
+            assert a == b
+
+
 def test_coarsest_pcg_persistent_many_blocks():
     """The persistent coarsest PCG where one lane polls several ring slots
     (G > 64 blocks): n_levels=2 on 48x24x24 puts a 24x12x12 coarsest level
