@@ -129,7 +129,7 @@ def parse():
                          "1 one persistent launch where it fits (default), 0 three launches")
     ap.add_argument("--fine-shapes", default="",
                     help="with --roofline-only (fp64): comma-separated launch shapes to time "
-                         "one after the other (1, 4, 8, 16, 72, 80; default: the autotuned one)")
+                         "one after the other (1, 4, 8, 16, 72, 80, 48, 50, 44; default: the autotuned one)")
     ap.add_argument("--roofline-kernel", default="fp64", choices=["fp64", "bf16", "stencil"],
                     help="with --roofline-only: the FP64 fine operator, its BF16 MFMA form, or "
                          "the level-1 coarse stencil rotated over set-up hierarchies")

@@ -119,8 +119,9 @@ int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* st
 int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape) {
   CHECK_PTR(ctx);
   if (shape != 0 && shape != 1 && shape != 4 && shape != 8 && shape != 16 &&
-      shape != kShapeMfma8 && shape != kShapeMfma16) {
-    set_error("gf_ctx_set_shape: shape must be 0, 1, 4, 8, 16, 72 or 80");
+      shape != kShapeMfma8 && shape != kShapeMfma16 && shape != kShapeBox488 &&
+      shape != kShapeBox2816 && shape != kShapeBox288) {
+    set_error("gf_ctx_set_shape: shape must be 0, 1, 4, 8, 16, 72, 80, 48, 50 or 44");
     return -2;
   }
   ctx->c.g.shape = shape;

@@ -148,16 +148,18 @@ __device__ __forceinline__ int map_idx(const DofMap& m, int dof) {
 // grid (in == false) or fixed read as 0.  The load always targets a valid
 // address (index 0 when masked), so every load of a window can be issued
 // back to back instead of behind an exec-mask branch.
+// The index is taken unsigned (a free index is below 2^31), so the load can
+// use a scalar base with a 32-bit lane offset and needs no sign extension.
 template <bool OFF, typename T>
 __device__ __forceinline__ T map_load(const DofMap& m, const T* __restrict__ u, int dof, bool in) {
   int f;
   if (OFF) {
     f = dof - m.off;
   } else {
-    f = m.f2f[in ? dof : 0];
+    f = m.f2f[in ? (unsigned)dof : 0u];
   }
   const bool ok = in & (f >= 0);
-  const T v = u[ok ? f : 0];
+  const T v = u[ok ? (unsigned)f : 0u];
   return ok ? v : T(0);
 }
 

@@ -447,6 +447,264 @@ __global__ __launch_bounds__(kBlock) void k_fine_matvec_mfma(Grid g, DofMap map,
   }
 }
 
+// Node-per-lane box form.  A block of BX*BY*BZ lanes owns a BX x BY x BZ box
+// of nodes (z fastest), one node per lane.  It stages the u values of the
+// (BX+2) x (BY+2) x (BZ+2) node window (a z-row's 3(BZ+2) DOFs as in memory,
+// rows RP entries apart) and the moduli of the (BX+1) x (BY+1) x (BZ+1) element window
+// in LDS; then each lane forms all 8 element contributions of its node with
+// the one-lane-per-node kernel's arithmetic: the 24-term row products in
+// column order as fused multiply-adds from zero, P_ab = E_ab1 t_ab1 +
+// E_ab0 t_ab0 in the contracted association, Q_a = P_a0 + P_a1, y = Q_0 + Q_1
+// -- bitwise identical to every other shape.  Against the four-wave tile a
+// node costs one lane instead of four: no cross-wave pair sums, one epilogue,
+// one window decomposition, and (BX+2)(BY+2)(BZ+2) / (BX BY BZ) staged nodes
+// per owned node (2.3 for 4x8x8) instead of 4.7.  A lane walks its 3x3x3
+// neighbourhood in x and holds at most two x-planes (54 values) at once.
+template <typename T>
+__device__ __forceinline__ T fmadd(T a, T b, T c);
+template <>
+__device__ __forceinline__ double fmadd<double>(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
+// P += E t for one element of the tree: FP64 contracted into one fused
+// multiply-add (as the other shapes compile `P += e * t`); FP32 the rounded
+// FP32 product added in FP64.
+__device__ __forceinline__ double tree_acc(double p, double e, double t) {
+  return __builtin_fma(e, t, p);
+}
+__device__ __forceinline__ double tree_acc(double p, float e, float t) {
+  return p + (double)__fmul_rn(e, t);
+}
+
+// Row products of element (a, b, c) from the x-planes a and a + 1 of the
+// node's 3x3x3 neighbourhood, w[px][yy][zz][q] (corner (px, py, pz) of the
+// element at yy = b + py, zz = c + pz), K the element's Ke rows 3L..3L+2:
+// the 24-term products in column order as fused multiply-adds from zero.
+template <typename T>
+__device__ __forceinline__ void box_rows(const T* __restrict__ K, const T (&w)[2][3][3][3], int b,
+                                         int c, T& t0, T& t1, T& t2) {
+  t0 = 0;
+  t1 = 0;
+  t2 = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int px = lpos(j, 0), py = lpos(j, 1), pz = lpos(j, 2);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T uu = w[px][b + py][c + pz][q];
+      t0 = fmadd<T>(K[3 * j + q], uu, t0);
+      t1 = fmadd<T>(K[24 + 3 * j + q], uu, t1);
+      t2 = fmadd<T>(K[48 + 3 * j + q], uu, t2);
+    }
+  }
+}
+
+// Stage the u window ((BX+2) x (BY+2) x (BZ+2) nodes; entry (xx, yy, zz, q)
+// at (xx (BY+2) + yy) RP + 3 zz + q) and the E window ((BX+1) x (BY+1) x
+// (BZ+1) elements, z-rows EZP entries apart) of the box at (x0, y0, z0).  A window
+// z-row is 3(BZ+2) consecutive DOFs; a thread keeps one position of a row.
+// Main pass: thread (yy, cu) loads row yy < TPU of every x-plane, so its
+// address moves by one constant plane stride per load and only the plane's
+// bounds test (uniform) changes.  The rows TPU..BY+1 of all planes follow as
+// a flat list of rows.  E: a thread keeps one (y, z) entry of an element
+// plane and takes the planes p, p + PPE, ...  All loads are issued before
+// the first LDS write; out-of-grid and fixed entries read 0.  Ends with a
+// barrier.
+template <typename T, bool OFF, int BX, int BY, int BZ, int RP, int EZP, int NT>
+__device__ __forceinline__ void stage_box(const Grid& g, const DofMap& map,
+                                          const T* __restrict__ E, const T* __restrict__ u,
+                                          T* us, T* es, int x0, int y0, int z0) {
+  constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2;
+  constexpr int RU = 3 * WZ;
+  constexpr int TPU = NT / RU < WY ? NT / RU : WY;  // rows per plane in the main pass
+  constexpr int RT = WY - TPU;                      // tail rows per plane
+  constexpr int NRT = WX * RT;                      // tail rows in all
+  constexpr int NTI = (NRT + TPU - 1) / TPU;
+  constexpr int EX = BX + 1, EY = BY + 1, EZ = BZ + 1;
+  constexpr int PLE = EY * EZ;                      // entries of an element plane
+  constexpr int PLP = EY * EZP;                     // its LDS extent (rows padded)
+  constexpr int PPE = NT / PLE;                     // element planes per pass
+  constexpr int NEI = (EX + PPE - 1) / PPE;
+  static_assert(TPU >= 1 && PPE >= 1, "box too small for its window rows");
+  const int plane_u = 3 * g.NZ * g.NY;
+  const int tu = threadIdx.x / RU, cu = threadIdx.x - tu * RU;
+  const int cz = cu / 3;
+  const int jz = z0 - 1 + cz;
+  const bool zin = (tu < TPU) & ((unsigned)jz < (unsigned)g.NZ);
+  const int dz = 3 * (z0 - 1) + cu;  // DOF offset of this position in a row
+  const int jy = y0 - 1 + tu;
+  const bool rin = zin & ((unsigned)jy < (unsigned)g.NY);
+  const int dmain = 3 * g.NZ * jy + dz + (x0 - 1) * plane_u;
+  T uv[WX], ut[NTI > 0 ? NTI : 1];
+#pragma unroll
+  for (int xx = 0; xx < WX; ++xx) {
+    const bool in = rin & ((unsigned)(x0 - 1 + xx) < (unsigned)g.NX);
+    uv[xx] = map_load<OFF>(map, u, dmain + xx * plane_u, in);
+  }
+#pragma unroll
+  for (int k = 0; k < NTI; ++k) {
+    const int r = tu + k * TPU;
+    const int xx = r / (RT > 0 ? RT : 1), yy = TPU + (r - xx * RT);
+    const int jx = x0 - 1 + xx, jyt = y0 - 1 + yy;
+    const bool in = zin & (r < NRT) & ((unsigned)jx < (unsigned)g.NX) &
+                    ((unsigned)jyt < (unsigned)g.NY);
+    ut[k] = map_load<OFF>(map, u, 3 * g.NZ * (jx * g.NY + jyt) + dz, in);
+  }
+  const int pe = threadIdx.x / PLE, re = threadIdx.x - pe * PLE;
+  const int ey = re / EZ, ez = re - ey * EZ;
+  const int cy = y0 - 1 + ey, czz = z0 - 1 + ez;
+  const bool yzin = (pe < PPE) & ((unsigned)cy < (unsigned)g.ny) & ((unsigned)czz < (unsigned)g.nz);
+  const int emain = ((x0 - 1 + pe) * g.ny + cy) * g.nz + czz;
+  T ev[NEI];
+#pragma unroll
+  for (int k = 0; k < NEI; ++k) {
+    const int p = pe + k * PPE;
+    const bool in = yzin & (p < EX) & ((unsigned)(x0 - 1 + p) < (unsigned)g.nx);
+    const T e = E[in ? emain + k * PPE * g.ny * g.nz : 0];
+    ev[k] = in ? e : T(0);
+  }
+  T* ub = us + cu;
+  if (tu < TPU) {
+#pragma unroll
+    for (int xx = 0; xx < WX; ++xx) ub[(xx * WY + tu) * RP] = uv[xx];
+  }
+#pragma unroll
+  for (int k = 0; k < NTI; ++k) {
+    const int r = tu + k * TPU;
+    const int xx = r / (RT > 0 ? RT : 1), yy = TPU + (r - xx * RT);
+    if ((tu < TPU) & (r < NRT)) ub[(xx * WY + yy) * RP] = ut[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NEI; ++k) {
+    const int p = pe + k * PPE;
+    if ((pe < PPE) & (p < EX)) es[p * PLP + ey * EZP + ez] = ev[k];
+  }
+  __syncthreads();
+}
+
+// The elements with a given a (the x side of the node) in a plane step: the
+// x-planes a and a + 1 of the neighbourhood are read from LDS (54 values) and
+// the four elements (a, b, c) are formed; returns
+// Q_a = (E_a00 t_a00 + E_a01 t_a01) + (E_a10 t_a10 + E_a11 t_a11) as the tree
+// has it.  Ke rows stream through the scalar registers (288 values).
+template <typename T, int WY, int RP, int EY, int EZ>
+__device__ __forceinline__ void box_plane_step(const T* __restrict__ Ke, const T* ub,
+                                               const T* eb, int a,
+                                               typename Acc<T>::type (&Qa)[3]) {
+  using A = typename Acc<T>::type;
+  const T* wb = ub + a * WY * RP;
+  T w[2][3][3][3];
+#pragma unroll
+  for (int px = 0; px < 2; ++px)
+#pragma unroll
+    for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+      for (int zz = 0; zz < 3; ++zz)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) w[px][yy][zz][q] = wb[(px * WY + yy) * RP + 3 * zz + q];
+  A P[2][3];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    P[b][0] = P[b][1] = P[b][2] = A(0);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const T e = eb[(a * EY + b) * EZ + c];
+      const int L = lidx(1 - a, 1 - b, 1 - c);
+      T t0, t1, t2;
+      box_rows<T>(Ke + 3 * L * 24, w, b, c, t0, t1, t2);
+      P[b][0] = tree_acc(P[b][0], e, t0);
+      P[b][1] = tree_acc(P[b][1], e, t1);
+      P[b][2] = tree_acc(P[b][2], e, t2);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) Qa[r] = P[0][r] + P[1][r];
+}
+
+// AS lanes per node: AS = 1 runs a = 0 and a = 1 as a loop that is not
+// unrolled (fully unrolled, the compiler hoists the loads of all 576 Ke
+// values and the whole neighbourhood, spilling scalar registers and halving
+// the waves per SIMD); AS = 2 gives a = 1 to a second group of waves, whose
+// Q_1 meets the first group's Q_0 in LDS (y = Q_0 + Q_1, the same sum) --
+// twice the waves for grids too small to fill the device one lane per node.
+template <typename T, int EPI, bool OFF, int BX, int BY, int BZ, int RP, int EZP, int AS>
+__global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofMap map,
+                                                                     const T* __restrict__ Ke,
+                                                                     const T* __restrict__ E,
+                                                                     const T* __restrict__ u,
+                                                                     T* __restrict__ y,
+                                                                     EpiArgs<T> ep,
+                                                                     const int* gate) {
+  using A = typename Acc<T>::type;
+  constexpr int NB = BX * BY * BZ;
+  constexpr int WY = BY + 2;
+  constexpr int EY = BY + 1;
+  static_assert(AS == 1 || (AS == 2 && NB % 64 == 0), "lane groups must be whole waves");
+  if (gate && *gate == 0) return;
+  __shared__ T us[(BX + 2) * WY * RP];
+  __shared__ T es[(BX + 1) * EY * EZP];
+  __shared__ A q1[AS == 2 ? 3 * NB : 1];
+  const int tiles_z = (g.NZ + BZ - 1) / BZ, tiles_y = (g.NY + BY - 1) / BY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, tx = tile / (tiles_z * tiles_y);
+  const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
+  stage_box<T, OFF, BX, BY, BZ, RP, EZP, AS * NB>(g, map, E, u, us, es, x0, y0, z0);
+  const int grp = AS == 2 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / NB) : 0;
+  const int t = threadIdx.x - grp * NB;
+  const int lz = t % BZ, ly = (t / BZ) % BY, lx = t / (BZ * BY);
+  const int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
+  const T* ub = us + (lx * WY + ly) * RP + 3 * lz;
+  const T* eb = es + (lx * EY + ly) * EZP + lz;
+  A ax[3];
+  int o0, o1, o2;
+  if constexpr (AS == 1) {
+    if ((ix >= g.NX) | (iy >= g.NY) | (iz >= g.NZ)) return;
+    const int n = (ix * g.NY + iy) * g.NZ + iz;
+    o0 = map_idx<OFF>(map, 3 * n);
+    o1 = map_idx<OFF>(map, 3 * n + 1);
+    o2 = map_idx<OFF>(map, 3 * n + 2);
+    if ((o0 & o1 & o2) < 0) return;
+    A Q0[3];
+#pragma unroll 1
+    for (int a = 0; a < 2; ++a) {
+      A Qa[3];
+      box_plane_step<T, WY, RP, EY, EZP>(Ke, ub, eb, a, Qa);
+      if (a == 0) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Q0[r] = Qa[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) ax[r] = Q0[r] + Qa[r];
+      }
+    }
+  } else {
+    const bool in = (ix < g.NX) & (iy < g.NY) & (iz < g.NZ);
+    const int n = in ? (ix * g.NY + iy) * g.NZ + iz : 0;
+    o0 = map_idx<OFF>(map, 3 * n);
+    o1 = map_idx<OFF>(map, 3 * n + 1);
+    o2 = map_idx<OFF>(map, 3 * n + 2);
+    const bool live = in & ((o0 & o1 & o2) >= 0);
+    A Qa[3] = {A(0), A(0), A(0)};
+    if (live) box_plane_step<T, WY, RP, EY, EZP>(Ke, ub, eb, grp, Qa);
+    if (grp == 1) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) q1[r * NB + t] = Qa[r];
+    }
+    __syncthreads();
+    if (grp == 1 || !live) return;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) ax[r] = Qa[r] + q1[r * NB + t];
+  }
+  const int o[3] = {o0, o1, o2};
+  const T axt[3] = {(T)ax[0], (T)ax[1], (T)ax[2]};
+  epilogue3<T, EPI>(o, axt, u, y, ep);
+}
+
 // Element strain energy and SIMP sensitivity (solver_v4.py:258-265):
 //   ce_e = u_e^T KE u_e  (unit modulus; KUe_j = sum_i u_i KE_ij in i order,
 //          then ce = sum_j KUe_j u_j in j order),
@@ -500,7 +758,8 @@ int elem_sensitivity(const Grid& g, const DofMap& map, const double* Ke, const d
 static std::atomic<int> g_split_f64{4}, g_split_f32{4};
 
 static bool valid_split(int s) {
-  return s == 1 || s == 4 || s == 8 || s == 16 || s == kShapeMfma8 || s == kShapeMfma16;
+  return s == 1 || s == 4 || s == 8 || s == 16 || s == kShapeMfma8 || s == kShapeMfma16 ||
+         s == kShapeBox488 || s == kShapeBox2816 || s == kShapeBox288;
 }
 
 void set_fine_split(int split_f64, int split_f32) {
@@ -509,10 +768,26 @@ void set_fine_split(int split_f64, int split_f32) {
 }
 int get_fine_split(int is_f64) { return is_f64 ? g_split_f64 : g_split_f32; }
 
+template <typename T, int EPI, bool OFF, int BX, int BY, int BZ, int AS>
+static void launch_box(const Grid& g, const DofMap& map, const T* Ke, const T* E, const T* u,
+                       T* y, const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
+  // LDS row pitches: with 8 nodes per z-row a 16-lane group spans two rows,
+  // which fall on disjoint banks when the pitch is 8 (mod 16) doubles (u
+  // rows: 30 -> 40; E rows: 9 -> 24); a 16-node row is conflict-free alone
+  constexpr int RP = BZ == 8 ? 40 : 3 * (BZ + 2), EZP = BZ == 8 ? 24 : BZ + 1;
+  const int tiles = cdiv(g.NX, BX) * cdiv(g.NY, BY) * cdiv(g.NZ, BZ);
+  hipLaunchKernelGGL((k_fine_matvec_box<T, EPI, OFF, BX, BY, BZ, RP, EZP, AS>), dim3(tiles),
+                     dim3(AS * BX * BY * BZ), 0, st, g, map, Ke, E, u, y, ep, gate);
+}
+
 template <typename T, int EPI, bool OFF>
 static void launch_fine_m(int split, const Grid& g, const DofMap& map, const T* Ke, const T* E,
                           const T* u, T* y, const EpiArgs<T>& ep, const int* gate,
                           hipStream_t st) {
+  if (split == kShapeBox488) return launch_box<T, EPI, OFF, 4, 8, 8, 1>(g, map, Ke, E, u, y, ep, gate, st);
+  if (split == kShapeBox2816)
+    return launch_box<T, EPI, OFF, 2, 8, 16, 1>(g, map, Ke, E, u, y, ep, gate, st);
+  if (split == kShapeBox288) return launch_box<T, EPI, OFF, 2, 8, 8, 2>(g, map, Ke, E, u, y, ep, gate, st);
   if (split == kShapeMfma8 || split == kShapeMfma16) {
     // matrix-core contraction for FP64; the FP32 operator takes the VALU
     // tile of the same patch

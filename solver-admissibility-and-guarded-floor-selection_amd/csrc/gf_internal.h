@@ -46,13 +46,16 @@ enum WaitFlag { WAIT_SWEEP = 0, WAIT_PCG = 1, WAIT_LS = 2, kWaitFlags = 3 };
 // Fine-operator launch shapes timed by the autotuner, in this order: 1 lane
 // per node, 4 waves per 64 nodes, LDS tiles 8x8 and 4x16, and the same tiles
 // with the FP64 contraction on the matrix cores (codes 72 and 80; the FP32
-// operator runs the VALU tile of the same patch there).
+// operator runs the VALU tile of the same patch there), then LDS boxes of
+// 4x8x8 and 2x8x16 nodes with one lane per node (codes 48, 50) and of 2x8x8
+// nodes with two lanes per node (code 44).
 constexpr int kShapeMfma8 = 72, kShapeMfma16 = 80;
+constexpr int kShapeBox488 = 48, kShapeBox2816 = 50, kShapeBox288 = 44;
 
 // Largest coarsest system of the two-triangular-solve form (its right-hand
 // side is held in LDS); the explicit-inverse form has no such bound.
 constexpr int kTrsvMaxN = 5000;
-constexpr int kFineShapes = 6;
+constexpr int kFineShapes = 9;
 
 // Epilogue applied by the operator kernels to each output DOF i:
 //   PLAIN : y_i = (A x)_i

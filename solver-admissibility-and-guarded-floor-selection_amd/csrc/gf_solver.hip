@@ -328,7 +328,15 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
   // the shapes are timed in three interleaved rounds and each keeps its best
   // round, so a burst of other work on the device (concurrent instances set
   // up at the same time) does not decide the choice
-  const int shapes[kFineShapes] = {1, 4, 8, 16, kShapeMfma8, kShapeMfma16};
+  const int shapes[kFineShapes] = {1,
+                                   4,
+                                   8,
+                                   16,
+                                   kShapeMfma8,
+                                   kShapeMfma16,
+                                   kShapeBox488,
+                                   kShapeBox2816,
+                                   kShapeBox288};
   float tmin[kFineShapes];
   for (int si = 0; si < kFineShapes; ++si) tmin[si] = 1e30f;
   for (int round = 0; round < 3; ++round) {

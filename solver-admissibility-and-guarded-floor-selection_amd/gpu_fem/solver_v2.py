@@ -102,11 +102,13 @@ class FineOperator:
         return self.mf_op.matvec(x, self.E)
 
 
-# fine-operator launch shapes in the autotuner's timing order: (name, code)
-# fine-operator launch shapes (gf_ctx_autotune order); the mfma shapes run the
-# FP64 element-row contraction on the matrix cores (FP32: the same VALU tile)
+# fine-operator launch shapes in the autotuner's timing order: (name, code);
+# the mfma shapes run the FP64 element-row contraction on the matrix cores
+# (FP32: the same VALU tile); the box shapes stage a 3D node box per block
+# and give each node one lane (box2x8x8s2: two)
 LAUNCH_SHAPES = (("node", 1), ("wave4", 4), ("tile8x8", 8), ("tile4x16", 16),
-                 ("mfma8x8", 72), ("mfma4x16", 80))
+                 ("mfma8x8", 72), ("mfma4x16", 80), ("box4x8x8", 48), ("box2x8x16", 50),
+                 ("box2x8x8s2", 44))
 
 
 class MatrixFreeKff:
@@ -143,7 +145,9 @@ class MatrixFreeKff:
 
     def set_launch_shape(self, shape: int):
         """Force a launch shape (0 default, 1 node, 4 wave4, 8/16 LDS tiles,
-        72/80 the LDS tiles with the FP64 contraction on the matrix cores)."""
+        72/80 the LDS tiles with the FP64 contraction on the matrix cores,
+        48/50 the boxes of 4x8x8 and 2x8x16 nodes with a lane per node, 44
+        the 2x8x8 box with two lanes per node)."""
         _lib.call("gf_ctx_set_shape", self._ctx.h, int(shape))
         self.launch_shape = int(shape)
 
