@@ -662,6 +662,7 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
   const T* eb = es + (lx * EY + ly) * EZP + lz;
   A ax[3];
   int o0, o1, o2;
+  EpiOps3<T> ev;  // epilogue operands, loaded before the arithmetic
   if constexpr (AS == 1) {
     if ((ix >= g.NX) | (iy >= g.NY) | (iz >= g.NZ)) return;
     const int n = (ix * g.NY + iy) * g.NZ + iz;
@@ -669,6 +670,8 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
     o1 = map_idx<OFF>(map, 3 * n + 1);
     o2 = map_idx<OFF>(map, 3 * n + 2);
     if ((o0 & o1 & o2) < 0) return;
+    const int o[3] = {o0, o1, o2};
+    epi_load3<T, EPI>(o, u, ep, ev);
     A Q0[3];
 #pragma unroll 1
     for (int a = 0; a < 2; ++a) {
@@ -689,6 +692,10 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
     o1 = map_idx<OFF>(map, 3 * n + 1);
     o2 = map_idx<OFF>(map, 3 * n + 2);
     const bool live = in & ((o0 & o1 & o2) >= 0);
+    if (grp == 0) {
+      const int o[3] = {o0, o1, o2};
+      epi_load3<T, EPI>(o, u, ep, ev);
+    }
     A Qa[3] = {A(0), A(0), A(0)};
     if (live) box_plane_step<T, WY, RP, EY, EZP>(Ke, ub, eb, grp, Qa);
     if (grp == 1) {
@@ -702,7 +709,7 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
   }
   const int o[3] = {o0, o1, o2};
   const T axt[3] = {(T)ax[0], (T)ax[1], (T)ax[2]};
-  epilogue3<T, EPI>(o, axt, u, y, ep);
+  epi_store3<T, EPI>(o, axt, ev, y, ep);
 }
 
 // Element strain energy and SIMP sensitivity (solver_v4.py:258-265):

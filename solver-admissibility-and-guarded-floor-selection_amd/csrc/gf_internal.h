@@ -136,22 +136,43 @@ __device__ __forceinline__ void epilogue1(int i, T ax, const T* __restrict__ x,
 // pointer may alias a later load through another as far as the compiler
 // knows, which otherwise serialises the three DOFs' loads.  The indices of
 // one node are distinct, so the early loads read what epilogue1 would.
+template <typename T>
+struct EpiOps3 {
+  T b[3], di[3], dv[3], xv[3];
+};
+
+// The epilogue operands of a node's three DOFs; they do not depend on the
+// operator result, so a kernel may load them before its arithmetic and let
+// their latency pass under it.
+template <typename T, int EPI>
+__device__ __forceinline__ void epi_load3(const int (&o)[3], const T* __restrict__ x,
+                                          const EpiArgs<T>& ep, EpiOps3<T>& v) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int i = o[q] >= 0 ? o[q] : 0;
+    v.b[q] = EPI != EPI_PLAIN ? ep.b[i] : T(0);
+    v.di[q] = epi_reads_x<EPI>() ? ep.dinv[i] : T(0);
+    v.xv[q] = epi_reads_x<EPI>() ? x[i] : T(0);
+    v.dv[q] = EPI == EPI_CHEB ? ep.dv[i] : T(0);
+  }
+}
+
+template <typename T, int EPI>
+__device__ __forceinline__ void epi_store3(const int (&o)[3], const T (&ax)[3],
+                                           const EpiOps3<T>& v, T* __restrict__ y,
+                                           const EpiArgs<T>& ep) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (o[q] >= 0) epi_store<T, EPI>(o[q], ax[q], v.b[q], v.di[q], v.dv[q], v.xv[q], y, ep);
+}
+
 template <typename T, int EPI>
 __device__ __forceinline__ void epilogue3(const int (&o)[3], const T (&ax)[3],
                                           const T* __restrict__ x, T* __restrict__ y,
                                           const EpiArgs<T>& ep) {
-  T b[3], di[3], dv[3], xv[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int i = o[q] >= 0 ? o[q] : 0;
-    b[q] = EPI != EPI_PLAIN ? ep.b[i] : T(0);
-    di[q] = epi_reads_x<EPI>() ? ep.dinv[i] : T(0);
-    xv[q] = epi_reads_x<EPI>() ? x[i] : T(0);
-    dv[q] = EPI == EPI_CHEB ? ep.dv[i] : T(0);
-  }
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-    if (o[q] >= 0) epi_store<T, EPI>(o[q], ax[q], b[q], di[q], dv[q], xv[q], y, ep);
+  EpiOps3<T> v;
+  epi_load3<T, EPI>(o, x, ep, v);
+  epi_store3<T, EPI>(o, ax, v, y, ep);
 }
 
 // ---- BLAS-1 (gf_blas.hip) --------------------------------------------------
