@@ -44,7 +44,11 @@ TRAFFIC_BYTES = 4320893
 # (profiles/coarse_stencil_80x40x20_roofline.md)
 STENCIL_TRAFFIC_BYTES = 19235215
 SHAPE_KERNELS = {1: "k_fine_matvec_node", 4: "k_fine_matvec_wave4",
-                 8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>"}
+                 8: "k_fine_matvec_tile<TZ=8>", 16: "k_fine_matvec_tile<TZ=16>",
+                 72: "k_fine_matvec_mfma<TZ=8>", 80: "k_fine_matvec_mfma<TZ=16>",
+                 48: "k_fine_matvec_box<4x8x8 nodes, 1 lane per node>",
+                 50: "k_fine_matvec_box<2x8x16 nodes, 1 lane per node>",
+                 44: "k_fine_matvec_box<2x8x8 nodes, 2 lanes per node>"}
 
 
 def progress(msg):
