@@ -421,10 +421,17 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
   const bool valid = n < gc.n_nodes;
   double acc[3] = {0.0, 0.0, 0.0};
   int o[3] = {-1, -1, -1};
+  double dinv[3] = {0.0, 0.0, 0.0};
   if (valid) {
     o[0] = mapc(3 * n);
     o[1] = mapc(3 * n + 1);
     o[2] = mapc(3 * n + 2);
+    // the zero start's diagonal is loaded before the children, so its
+    // latency passes under theirs
+    if (zs.mode != 0 && grp == 0) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) dinv[q] = zs.dinv[o[q] >= 0 ? o[q] : 0];
+    }
     int ix, iy, iz;
     node_xyz(gc, n, ix, iy, iz);
     double wx[3], wy[3], wz[3];
@@ -467,11 +474,11 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
     if (i >= 0) {
       yc[i] = v;
       if (zs.mode == 1) {
-        const double z = rn_mul(zs.dinv[i], v);
+        const double z = rn_mul(dinv[q], v);
         zs.dv[i] = z;
         zs.xo[i] = rn_add(0.0, rn_mul(zs.a, z));
       } else if (zs.mode == 2) {
-        zs.xo[i] = rn_add(0.0, rn_mul(rn_mul(zs.a, zs.dinv[i]), v));
+        zs.xo[i] = rn_add(0.0, rn_mul(rn_mul(zs.a, dinv[q]), v));
       }
     }
   }
