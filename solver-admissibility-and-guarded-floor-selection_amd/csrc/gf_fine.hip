@@ -588,32 +588,33 @@ __device__ __forceinline__ void stage_box(const Grid& g, const DofMap& map,
 }
 
 // The elements with a given a (the x side of the node) in a plane step: the
-// x-planes a and a + 1 of the neighbourhood are read from LDS (54 values) and
-// the four elements (a, b, c) are formed; returns
+// x-planes a and a + 1 of the neighbourhood are read from LDS (p0 and p1 at
+// the lane's (y, z) window entry; 54 values) with the element plane a (pe at
+// the lane's entry) and the four elements (a, b, c) are formed; returns
 // Q_a = (E_a00 t_a00 + E_a01 t_a01) + (E_a10 t_a10 + E_a11 t_a11) as the tree
 // has it.  Ke rows stream through the scalar registers (288 values).
-template <typename T, int WY, int RP, int EY, int EZ>
-__device__ __forceinline__ void box_plane_step(const T* __restrict__ Ke, const T* ub,
-                                               const T* eb, int a,
+template <typename T, int RP, int EZ>
+__device__ __forceinline__ void box_plane_step(const T* __restrict__ Ke, const T* p0, const T* p1,
+                                               const T* pe, int a,
                                                typename Acc<T>::type (&Qa)[3]) {
   using A = typename Acc<T>::type;
-  const T* wb = ub + a * WY * RP;
   T w[2][3][3][3];
 #pragma unroll
-  for (int px = 0; px < 2; ++px)
+  for (int yy = 0; yy < 3; ++yy)
 #pragma unroll
-    for (int yy = 0; yy < 3; ++yy)
+    for (int zz = 0; zz < 3; ++zz)
 #pragma unroll
-      for (int zz = 0; zz < 3; ++zz)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) w[px][yy][zz][q] = wb[(px * WY + yy) * RP + 3 * zz + q];
+      for (int q = 0; q < 3; ++q) {
+        w[0][yy][zz][q] = p0[yy * RP + 3 * zz + q];
+        w[1][yy][zz][q] = p1[yy * RP + 3 * zz + q];
+      }
   A P[2][3];
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     P[b][0] = P[b][1] = P[b][2] = A(0);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const T e = eb[(a * EY + b) * EZ + c];
+      const T e = pe[b * EZ + c];
       const int L = lidx(1 - a, 1 - b, 1 - c);
       T t0, t1, t2;
       box_rows<T>(Ke + 3 * L * 24, w, b, c, t0, t1, t2);
@@ -676,7 +677,8 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
 #pragma unroll 1
     for (int a = 0; a < 2; ++a) {
       A Qa[3];
-      box_plane_step<T, WY, RP, EY, EZP>(Ke, ub, eb, a, Qa);
+      box_plane_step<T, RP, EZP>(Ke, ub + a * WY * RP, ub + (a + 1) * WY * RP,
+                                 eb + a * EY * EZP, a, Qa);
       if (a == 0) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) Q0[r] = Qa[r];
@@ -697,7 +699,9 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
       epi_load3<T, EPI>(o, u, ep, ev);
     }
     A Qa[3] = {A(0), A(0), A(0)};
-    if (live) box_plane_step<T, WY, RP, EY, EZP>(Ke, ub, eb, grp, Qa);
+    if (live)
+      box_plane_step<T, RP, EZP>(Ke, ub + grp * WY * RP, ub + (grp + 1) * WY * RP,
+                                 eb + grp * EY * EZP, grp, Qa);
     if (grp == 1) {
 #pragma unroll
       for (int r = 0; r < 3; ++r) q1[r * NB + t] = Qa[r];
