@@ -212,10 +212,9 @@ __global__ __launch_bounds__(64 * NW) void k_stencil_nodes(Grid g, DofMap map,
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const T xv = map_load<OFF>(map, x, 3 * m + q, in);
-      // coefficients are read once per application: streamed past the caches
-      s0 += __builtin_nontemporal_load(&A[(k * 9 + 0 * 3 + q) * nn + nn_]) * xv;
-      s1 += __builtin_nontemporal_load(&A[(k * 9 + 1 * 3 + q) * nn + nn_]) * xv;
-      s2 += __builtin_nontemporal_load(&A[(k * 9 + 2 * 3 + q) * nn + nn_]) * xv;
+      s0 += A[(k * 9 + 0 * 3 + q) * nn + nn_] * xv;
+      s1 += A[(k * 9 + 1 * 3 + q) * nn + nn_] * xv;
+      s2 += A[(k * 9 + 2 * 3 + q) * nn + nn_] * xv;
     }
     if (kin) {
       tk[k][0][ln] = s0;
