@@ -558,7 +558,8 @@ __device__ __forceinline__ void stage_box(const Grid& g, const DofMap& map,
   const int pe = threadIdx.x / PLE, re = threadIdx.x - pe * PLE;
   const int ey = re / EZ, ez = re - ey * EZ;
   const int cy = y0 - 1 + ey, czz = z0 - 1 + ez;
-  const bool yzin = (pe < PPE) & ((unsigned)cy < (unsigned)g.ny) & ((unsigned)czz < (unsigned)g.nz);
+  const bool yzin =
+      (pe < PPE) & ((unsigned)cy < (unsigned)g.ny) & ((unsigned)czz < (unsigned)g.nz);
   const int emain = ((x0 - 1 + pe) * g.ny + cy) * g.nz + czz;
   T ev[NEI];
 #pragma unroll
@@ -766,7 +767,9 @@ int elem_sensitivity(const Grid& g, const DofMap& map, const double* Ke, const d
   return 0;
 }
 
-static std::atomic<int> g_split_f64{4}, g_split_f32{4};
+// launch shape of contexts that were not autotuned: the 2x8x8 box, the
+// autotuner's choice at 64k, 512k and 1M elements
+static std::atomic<int> g_split_f64{kShapeBox288}, g_split_f32{kShapeBox288};
 
 static bool valid_split(int s) {
   return s == 1 || s == 4 || s == 8 || s == 16 || s == kShapeMfma8 || s == kShapeMfma16 ||
@@ -795,10 +798,12 @@ template <typename T, int EPI, bool OFF>
 static void launch_fine_m(int split, const Grid& g, const DofMap& map, const T* Ke, const T* E,
                           const T* u, T* y, const EpiArgs<T>& ep, const int* gate,
                           hipStream_t st) {
-  if (split == kShapeBox488) return launch_box<T, EPI, OFF, 4, 8, 8, 1>(g, map, Ke, E, u, y, ep, gate, st);
+  if (split == kShapeBox488)
+    return launch_box<T, EPI, OFF, 4, 8, 8, 1>(g, map, Ke, E, u, y, ep, gate, st);
   if (split == kShapeBox2816)
     return launch_box<T, EPI, OFF, 2, 8, 16, 1>(g, map, Ke, E, u, y, ep, gate, st);
-  if (split == kShapeBox288) return launch_box<T, EPI, OFF, 2, 8, 8, 2>(g, map, Ke, E, u, y, ep, gate, st);
+  if (split == kShapeBox288)
+    return launch_box<T, EPI, OFF, 2, 8, 8, 2>(g, map, Ke, E, u, y, ep, gate, st);
   if (split == kShapeMfma8 || split == kShapeMfma16) {
     // matrix-core contraction for FP64; the FP32 operator takes the VALU
     // tile of the same patch
