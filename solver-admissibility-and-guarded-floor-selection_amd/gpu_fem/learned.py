@@ -671,11 +671,56 @@ def calibrate_gates(model: "FloorCNN", rows, target: float = 0.99) -> dict:
     return gates
 
 
+def evaluate(model: FloorCNN, states, *, workers: int = 12, load: str = "edge", progress=None,
+             chunk: int = 16, **kw) -> dict:
+    """Fresh-state evaluation: the plain FP64 policy (the reference floor),
+    then the learned policy with FP32 multigrid ungated and with the
+    network's per-grid confidence gates, all on ``workers`` concurrent
+    instances.  Returns the comparison of each learned run with the plain
+    one, over all states and per grid, with accepted solves/s."""
+    from .batch import precision_stack
+
+    dens = [density_of(s) for s in states]
+
+    def acc(rows):
+        return sum(int(r["solve_converged"]) for r in rows)
+
+    _sync()
+    t0 = time.perf_counter()
+    plain = label_states(states, workers=workers, load=load, densities=dens,
+                         progress=progress, chunk=chunk, **kw)
+    _sync()
+    w_plain = time.perf_counter() - t0
+    out = {"states": len(states),
+           "grids": sorted({"x".join(map(str, s["dims"])) for s in states}),
+           "plain_fp64": {"accepted": acc(plain), "wall_s": w_plain,
+                          "value": acc(plain) / w_plain}}
+    groups = _groups(states)
+    for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", "auto")):
+        got, wall = run_learned(model, states, workers=workers, load=load, densities=dens,
+                                progress=progress, chunk=chunk, confidence_threshold=gate,
+                                **kw, **precision_stack("fp32"))
+        c = compare(plain, got)
+        c.update(value=acc(got) / wall, wall_s=wall,
+                 speedup_vs_plain_fp64_concurrent=(acc(got) / wall) / (acc(plain) / w_plain),
+                 mean_total_iters=float(np.mean([r["total_iters"] for r in got])),
+                 plain_mean_total_iters=float(np.mean(
+                     [len(r["probe_history"]) - 1
+                      + sum(len(h) - 1 for _, h in r["solve_history"]) for r in plain])))
+        c["per_grid"] = {"x".join(map(str, d)): compare([plain[i] for i in idx],
+                                                       [got[i] for i in idx])
+                         for d, idx in groups.items()}
+        out[name] = c
+    return out
+
+
 def main(argv=None):
     """Label states with the plain policy, or train the network.
 
     python -m gpu_fem.learned label --grids 160x80x40 --states 96 --seed 5 --out labels.csv
     python -m gpu_fem.learned train --labels a.csv,b.csv --out floor_cnn.npz
+    python -m gpu_fem.learned evaluate --model floor_cnn.npz --grids 80x40x20 --states 96 \
+        --seed 11 --out eval.json
 
     ``train`` holds out every seventh state of each grid to calibrate the
     per-grid confidence gates, trains on the rest (Adam, BF16 kernels) and
@@ -694,6 +739,14 @@ def main(argv=None):
     la.add_argument("--workers", type=int, default=8)
     la.add_argument("--penal", type=float, default=3.0)
     la.add_argument("--out", required=True)
+    ev = sub.add_parser("evaluate")
+    ev.add_argument("--model", required=True)
+    ev.add_argument("--grids", required=True)
+    ev.add_argument("--states", type=int, required=True)
+    ev.add_argument("--seed", type=int, required=True)
+    ev.add_argument("--workers", type=int, default=12)
+    ev.add_argument("--penal", type=float, default=3.0)
+    ev.add_argument("--out", required=True)
     tr = sub.add_parser("train")
     tr.add_argument("--labels", required=True)
     tr.add_argument("--epochs", type=int, default=60)
@@ -710,6 +763,19 @@ def main(argv=None):
         say(f"labelling {len(states)} states on {grids}")
         rows = label_states(states, workers=a.workers, progress=say, chunk=16, penal=a.penal)
         write_labels(a.out, states, rows)
+        say(f"wrote {a.out}")
+        return
+    if a.cmd == "evaluate":
+        import json
+
+        grids = [tuple(int(v) for v in g.split("x")) for g in a.grids.split(",") if g]
+        states = sample_states(a.states, a.seed, grids)
+        model = FloorCNN.load(a.model)
+        say(f"evaluating {a.model} on {len(states)} fresh states on {grids}")
+        res = evaluate(model, states, workers=a.workers, progress=say, penal=a.penal)
+        res.update(model=a.model, seed=a.seed)
+        with open(a.out, "w") as f:
+            json.dump(res, f, indent=1)
         say(f"wrote {a.out}")
         return
     states, rows = [], []

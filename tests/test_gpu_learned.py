@@ -235,3 +235,27 @@ def test_learned_form_at_512k():
         assert c["max_accepted_true_rel_residual"] <= 1e-6
         if gate == 0.0:
             assert iters < plain_iters
+
+
+def test_evaluate_reports_agreement_against_the_plain_policy():
+    """The fresh-state evaluation runs the plain policy and both learned
+    forms on the same states and compares them: every row of the learned
+    runs is counted, the per-grid comparison adds up to the overall one,
+    and proposing the plain floor state by state reproduces agreement 1."""
+    from gpu_fem import learned as L
+
+    model = L.FloorCNN.load()
+    states = L.sample_states(4, 31, [(40, 20, 20), (20, 10, 10)])
+    res = L.evaluate(model, states, workers=2)
+    assert res["states"] == 4 and res["plain_fp64"]["accepted"] >= 1
+    for name in ("fp32_ungated", "fp32_gated"):
+        c = res[name]
+        assert c["states"] == 4
+        assert sum(g["states"] for g in c["per_grid"].values()) == 4
+        agree = sum(g["floor_agreement"] * g["states"] for g in c["per_grid"].values())
+        assert abs(agree - c["floor_agreement"] * 4) < 1e-9
+        # a grid outside the training range always takes the plain probe
+        small = c["per_grid"].get("20x10x10")
+        if small is not None:
+            assert small["out_of_range_rate"] == 1.0 and small["floor_agreement"] == 1.0
+        assert c["max_accepted_true_rel_residual"] <= 1e-6
