@@ -671,15 +671,15 @@ def calibrate_gates(model: "FloorCNN", rows, target: float = 0.99) -> dict:
     return gates
 
 
-def evaluate(model: FloorCNN, states, *, workers: int = 12, load: str = "edge", progress=None,
+def evaluate(model, states, *, workers: int = 12, load: str = "edge", progress=None,
              chunk: int = 16, **kw) -> dict:
     """Fresh-state evaluation: the plain FP64 policy (the reference floor),
     then the learned policy with FP32 multigrid ungated and with the
     network's per-grid confidence gates, all on ``workers`` concurrent
     instances.  Returns the comparison of each learned run with the plain
-    one, over all states and per grid, with accepted solves/s."""
-    from .batch import precision_stack
-
+    one, over all states and per grid, with accepted solves/s.  ``model``
+    may be a dict of named networks evaluated on the same plain run; the
+    result then holds one such comparison per name."""
     dens = [density_of(s) for s in states]
 
     def acc(rows):
@@ -696,6 +696,26 @@ def evaluate(model: FloorCNN, states, *, workers: int = 12, load: str = "edge", 
            "plain_fp64": {"accepted": acc(plain), "wall_s": w_plain,
                           "value": acc(plain) / w_plain}}
     groups = _groups(states)
+    if isinstance(model, dict):
+        for mname, m in model.items():
+            if progress is not None:
+                progress(f"evaluating {mname}")
+            out[mname] = _evaluate_learned(m, states, dens, plain, w_plain, groups, workers,
+                                           load, progress, chunk, kw)
+        return out
+    out.update(_evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load,
+                                 progress, chunk, kw))
+    return out
+
+
+def _evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load, progress,
+                      chunk, kw) -> dict:
+    from .batch import precision_stack
+
+    def acc(rows):
+        return sum(int(r["solve_converged"]) for r in rows)
+
+    out = {}
     for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", "auto")):
         got, wall = run_learned(model, states, workers=workers, load=load, densities=dens,
                                 progress=progress, chunk=chunk, confidence_threshold=gate,
@@ -770,7 +790,9 @@ def main(argv=None):
 
         grids = [tuple(int(v) for v in g.split("x")) for g in a.grids.split(",") if g]
         states = sample_states(a.states, a.seed, grids)
-        model = FloorCNN.load(a.model)
+        paths = [m for m in a.model.split(",") if m]
+        model = (FloorCNN.load(paths[0]) if len(paths) == 1
+                 else {m: FloorCNN.load(m) for m in paths})
         say(f"evaluating {a.model} on {len(states)} fresh states on {grids}")
         res = evaluate(model, states, workers=a.workers, progress=say, penal=a.penal)
         res.update(model=a.model, seed=a.seed)
