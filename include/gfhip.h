@@ -142,11 +142,13 @@ int32_t gf_ctx_destroy(gf_ctx* ctx);
  * 50 a node per lane with the u and E windows of a 4x8x8 or 2x8x16 (x,y,z)
  * node box staged in LDS, 44 a 2x8x8 box with two lanes per node (one per
  * x side of the node's elements) -- and keeps the fastest for every later call on ctx (each shape's best of three
- * interleaved timing rounds).  All shapes give bitwise-identical
- * results.  us_out (nullable, host): 9 floats of timings in that order;
+ * interleaved timing rounds; the FP32 operator's shape is chosen the same
+ * way from its own timings).  All shapes give bitwise-identical
+ * results.  us_out (nullable, host): 9 floats of FP64 timings in that order;
  * shape_out (nullable, host): the chosen shape.  Synchronises. */
 int32_t gf_ctx_autotune(gf_ctx* ctx, float* us_out, int32_t* shape_out, void* stream);
-/* Force a launch shape: 0 = process default, 1, 4, 8, 16, 72, 80, 48, 50 or 44. */
+/* Force a launch shape (FP64 and FP32 operators): 0 = process default, 1, 4,
+ * 8, 16, 72, 80, 48, 50 or 44. */
 int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape);
 /* Process-wide launch form of the coarse-level (27-point block stencil)
  * operator kernels: 1 = node-per-lane workgroups of eight waves with

@@ -125,6 +125,7 @@ int32_t gf_ctx_set_shape(gf_ctx* ctx, int32_t shape) {
     return -2;
   }
   ctx->c.g.shape = shape;
+  ctx->c.g.shape32 = 0;  // a forced shape holds for both precisions
   return 0;
 }
 

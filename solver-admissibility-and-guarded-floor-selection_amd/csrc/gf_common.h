@@ -64,6 +64,7 @@ struct Grid {
   int n_nodes;
   int ndof;
   int shape;           // fine-operator launch shape (0 = process default)
+  int shape32;         // its FP32 operator's shape (0 = as shape)
 };
 
 inline Grid make_grid(int nx, int ny, int nz) {
@@ -73,6 +74,7 @@ inline Grid make_grid(int nx, int ny, int nz) {
   g.n_nodes = g.NX * g.NY * g.NZ;
   g.ndof = 3 * g.n_nodes;
   g.shape = 0;
+  g.shape32 = 0;
   return g;
 }
 

@@ -850,7 +850,9 @@ static void launch_fine(int split, const Grid& g, const DofMap& map, const T* Ke
 template <typename T>
 int fine_matvec(const Grid& g, const DofMap& map, const T* Ke, const T* E, const T* u, T* y,
                 int epi, const EpiArgs<T>& ep, const int* gate, hipStream_t st) {
-  const int split = g.shape ? g.shape : get_fine_split(sizeof(T) == 8);
+  const int split = (sizeof(T) == 4 && g.shape32) ? g.shape32
+                    : g.shape                       ? g.shape
+                                                    : get_fine_split(sizeof(T) == 8);
   switch (epi) {
     case EPI_PLAIN: launch_fine<T, EPI_PLAIN>(split, g, map, Ke, E, u, y, ep, gate, st); break;
     case EPI_RESID: launch_fine<T, EPI_RESID>(split, g, map, Ke, E, u, y, ep, gate, st); break;

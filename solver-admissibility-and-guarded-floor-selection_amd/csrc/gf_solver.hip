@@ -368,6 +368,44 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
     }
   }
   g.shape = best_shape;
+  // the FP32 operator (multigrid smoothers of the FP32 stacks) is timed on
+  // its own: its best shape can differ (one lane per node in a 4x8x8 box at
+  // 1M elements); the double buffers serve as float buffers of the same sizes
+  {
+    const float* E32 = reinterpret_cast<const float*>(E);
+    const float* u32 = reinterpret_cast<const float*>(u);
+    float* y32 = reinterpret_cast<float*>(y);
+    float t32[kFineShapes];
+    for (int si = 0; si < kFineShapes; ++si) t32[si] = 1e30f;
+    for (int round = 0; round < 3; ++round) {
+      for (int si = 0; si < kFineShapes; ++si) {
+        Grid gg = g;
+        gg.shape = shapes[si];
+        gg.shape32 = 0;
+        for (int w = 0; w < 3; ++w)
+          GF_TRY(fine_matvec<float>(gg, map(), Ke32, E32, u32, y32, EPI_PLAIN, EpiArgs<float>{},
+                                    nullptr, st));
+        GF_HIP(hipEventRecord(e0, st));
+        const int reps = 20;
+        for (int r = 0; r < reps; ++r)
+          GF_TRY(fine_matvec<float>(gg, map(), Ke32, E32, u32, y32, EPI_PLAIN, EpiArgs<float>{},
+                                    nullptr, st));
+        GF_HIP(hipEventRecord(e1, st));
+        GF_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        GF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        t32[si] = std::min(t32[si], ms * 1000.f / reps);
+      }
+    }
+    int best32 = best_shape;
+    float b32 = 1e30f;
+    for (int si = 0; si < kFineShapes; ++si)
+      if (t32[si] < b32) {
+        b32 = t32[si];
+        best32 = shapes[si];
+      }
+    g.shape32 = best32;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   dfree(E);
