@@ -409,6 +409,20 @@ __device__ __forceinline__ double ls_fold(const double* parts, int G, double* ld
   return block_sum(a, lds);
 }
 
+// ls_fold of NP partial arrays (kRedBlocks apart) at once; results in thread 0
+template <int NP>
+__device__ __forceinline__ void ls_fold_n(const double* parts, int G, double (&out)[NP],
+                                          double* lds) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    out[p] = 0.0;
+    for (int i = threadIdx.x; i < G; i += blockDim.x)
+      out[p] += __hip_atomic_load(&parts[p * kRedBlocks + i], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum_n<NP>(out, lds);
+}
+
 __global__ __launch_bounds__(kBlock) void k_ls_fused(long n, const double* __restrict__ c,
                                                      const double* __restrict__ r,
                                                      const double* __restrict__ Ac,
@@ -417,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_ls_fused(long n, const double* __res
                                                      double* parts, unsigned long long* cnt,
                                                      double* err, const int* gate) {
   if (gate && *gate == 0) return;  // uniform: written by an earlier kernel
-  __shared__ double lds[4];
+  __shared__ double lds[16];
   __shared__ double sst[8];
   __shared__ unsigned long long base_s;
   const int G = gridDim.x, b = blockIdx.x;
@@ -453,24 +467,25 @@ __global__ __launch_bounds__(kBlock) void k_ls_fused(long n, const double* __res
       a[3] += rv * av;
     }
   }
-  double bs[4];
-#pragma unroll
-  for (int p = 0; p < 4; ++p) bs[p] = block_sum(a[p], lds);
-  const unsigned long long base = base_s;  // read before the first barrier of block_sum
+  block_sum_n<4>(a, lds);
+  const unsigned long long base = base_s;  // written by thread 0 before block_sum_n's barriers
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int p = 0; p < 4; ++p)
-      __hip_atomic_store(&parts[p * kRedBlocks + b], bs[p], __ATOMIC_RELAXED,
+      __hip_atomic_store(&parts[p * kRedBlocks + b], a[p], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     ls_arrive_wait(cnt, base + (unsigned long long)G, err);
   }
   __syncthreads();
+  {
+    double t[4];
+    ls_fold_n<4>(parts, G, t, lds);
+    if (threadIdx.x == 0) {
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const double t = ls_fold(parts + p * kRedBlocks, G, lds);
-    if (threadIdx.x == 0) sst[p] = t;
+      for (int p = 0; p < 4; ++p) sst[p] = t[p];
+      ls_scale(sst, max_scale);
+    }
   }
-  if (threadIdx.x == 0) ls_scale(sst, max_scale);
   __syncthreads();
   // phase 2: y = r - alpha Ac and ||y||^2 (k_trial_norm's loop and order)
   const double s = sst[4];

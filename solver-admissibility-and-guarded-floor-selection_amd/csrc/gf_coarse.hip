@@ -342,50 +342,61 @@ int f64_to_f32_block(long n, const double* A, float* A32, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // trilinear transfers (row-normalised scalar weights, applied per component)
 // ---------------------------------------------------------------------------
-// parents of fine index i along one axis: up to 2 (index, weight) pairs, in
-// ascending index order, with the 1-D row sum (a power of two).
-__device__ __forceinline__ int parents_1d(int i, int ncn, int* p, double* w, double& s) {
-  int cnt = 0;
-  s = 0.0;
-  if ((i & 1) == 0) {
-    if (i / 2 < ncn) { p[cnt] = i / 2; w[cnt] = 1.0; ++cnt; }
-  } else {
-    if (i / 2 < ncn) { p[cnt] = i / 2; w[cnt] = 0.5; ++cnt; }
-    if (i / 2 + 1 < ncn) { p[cnt] = i / 2 + 1; w[cnt] = 0.5; ++cnt; }
-  }
-  for (int c = 0; c < cnt; ++c) s += w[c];
-  return cnt;
-}
-
+// Row-normalised trilinear weight of coarse parent I for fine index i >= 0:
+// a parent's 1-D weight (1 for the one parent of an even index, 1/2 for each
+// of the two of an odd one) divided by the row's sum, in closed form.  An
+// odd index past the last coarse node keeps one parent, 1/2 / 1/2 = 1.
+// Every quotient is exact (1, 1/2 or 0), so no division is needed.
 __device__ __forceinline__ double pweight(int i, int I, int ncn) {
-  int p[2];
-  double w[2], s;
-  const int cnt = parents_1d(i, ncn, p, w, s);
-  for (int c = 0; c < cnt; ++c)
-    if (p[c] == I) return w[c] / s;
-  return 0.0;
+  const int h = i >> 1;
+  const bool lo = h < ncn, hi = h + 1 < ncn;
+  if ((i & 1) == 0) return (I == h && lo) ? 1.0 : 0.0;
+  if (I == h && lo) return hi ? 0.5 : 1.0;
+  return (I == h + 1 && hi) ? 0.5 : 0.0;
 }
 
+// Parents of fine index i >= 0 along one axis with their normalised weights
+// (pweight of each), in ascending index order; the count is 0, 1 or 2.
+__device__ __forceinline__ int parents_norm(int i, int ncn, int* p, double* w) {
+  const int h = i >> 1;
+  const bool lo = h < ncn, hi = h + 1 < ncn;
+  p[0] = h;
+  p[1] = h + 1;
+  if ((i & 1) == 0) {
+    w[0] = 1.0;
+    w[1] = 0.0;
+    return lo ? 1 : 0;
+  }
+  w[0] = hi ? 0.5 : 1.0;
+  w[1] = 0.5;
+  return lo ? (hi ? 2 : 1) : 0;
+}
+
+// Prolongation: fine node n gathers its up to 2x2x2 coarse parents.  The
+// map modes of both levels are template parameters (OFF: free index = DOF -
+// off), so an offset-mapped level costs no table load and no branch.
+template <bool OFFF, bool OFFC>
 __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
                                                     const double* __restrict__ ec,
                                                     double* __restrict__ yf, const int* gate) {
   if (gate && *gate == 0) return;
   const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= gf.n_nodes) return;
-  const int o0 = mapf(3 * n), o1 = mapf(3 * n + 1), o2 = mapf(3 * n + 2);
+  const int o0 = map_idx<OFFF>(mapf, 3 * n), o1 = map_idx<OFFF>(mapf, 3 * n + 1),
+            o2 = map_idx<OFFF>(mapf, 3 * n + 2);
   if ((o0 & o1 & o2) < 0) return;
   int ix, iy, iz;
   node_xyz(gf, n, ix, iy, iz);
-  int px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
-  double wx[2] = {0.0, 0.0}, wy[2] = {0.0, 0.0}, wz[2] = {0.0, 0.0}, sx, sy, sz;
-  const int cx = parents_1d(ix, gc.NX, px, wx, sx);
-  const int cy = parents_1d(iy, gc.NY, py, wy, sy);
-  const int cz = parents_1d(iz, gc.NZ, pz, wz, sz);
-  const double inv = 1.0 / ((sx * sy) * sz);
+  int px[2], py[2], pz[2];
+  double wx[2], wy[2], wz[2];
+  const int cx = parents_norm(ix, gc.NX, px, wx);
+  const int cy = parents_norm(iy, gc.NY, py, wy);
+  const int cz = parents_norm(iz, gc.NZ, pz, wz);
   double acc[3] = {0.0, 0.0, 0.0};
   // all 2x2x2 parent slots with a fixed trip count (slots past a parent
-  // count are masked), so the up to 24 loads issue together; the terms are
-  // added in the same (a, b, c) order as before
+  // count are masked and read index 0), so the up to 24 loads issue
+  // together; terms are added in (a, b, c) order.  Every weight is a power
+  // of two, so ((wx wy) wz) is the exact normalised product.
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -393,12 +404,12 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid g
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const bool in = (a < cx) & (b < cy) & (c < cz);
-        const double w = ((wx[a] * wy[b]) * wz[c]) * inv;
-        const int m = (px[a] * gc.NY + py[b]) * gc.NZ + pz[c];
+        const double w = (wx[a] * wy[b]) * wz[c];
+        const int m = in ? (px[a] * gc.NY + py[b]) * gc.NZ + pz[c] : 0;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          const int f = in ? mapc(3 * m + q) : -1;
-          const double v = ec[f >= 0 ? f : 0];
+          const int f = in ? map_idx<OFFC>(mapc, 3 * m + q) : -1;
+          const double v = ec[f >= 0 ? (unsigned)f : 0u];
           if (f >= 0) acc[q] += w * v;
         }
       }
@@ -409,7 +420,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid g
 
 // Restriction R = P^T: coarse node I gathers its (up to 27) fine children in
 // ascending fine-node order.  64 coarse nodes x 4 child groups per block,
-// partial sums combined in LDS in group order.
+// partial sums combined in LDS in group order.  Map modes as for k_prolong.
+template <bool OFFF, bool OFFC>
 __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
                                                      const double* __restrict__ rf,
                                                      double* __restrict__ yc, ZeroStart zs,
@@ -423,14 +435,14 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
   int o[3] = {-1, -1, -1};
   double dinv[3] = {0.0, 0.0, 0.0};
   if (valid) {
-    o[0] = mapc(3 * n);
-    o[1] = mapc(3 * n + 1);
-    o[2] = mapc(3 * n + 2);
+    o[0] = map_idx<OFFC>(mapc, 3 * n);
+    o[1] = map_idx<OFFC>(mapc, 3 * n + 1);
+    o[2] = map_idx<OFFC>(mapc, 3 * n + 2);
     // the zero start's diagonal is loaded before the children, so its
     // latency passes under theirs
     if (zs.mode != 0 && grp == 0) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) dinv[q] = zs.dinv[o[q] >= 0 ? o[q] : 0];
+      for (int q = 0; q < 3; ++q) dinv[q] = zs.dinv[o[q] >= 0 ? (unsigned)o[q] : 0u];
     }
     int ix, iy, iz;
     node_xyz(gc, n, ix, iy, iz);
@@ -443,8 +455,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
       wz[a] = (fz >= 0 && fz < gf.NZ) ? pweight(fz, iz, gc.NZ) : 0.0;
     }
     // the group's children c27 = grp, grp+4, ..., grp+24 (the last for
-    // grp < 3 only) with a fixed trip count, absent children masked, so the
-    // loads issue together; terms are added in the same order as before
+    // grp < 3 only) with a fixed trip count, absent children masked and
+    // reading index 0, so the loads issue together; terms are added in
+    // ascending child order
 #pragma unroll
     for (int t = 0; t < 7; ++t) {
       const bool kin = (t < 6) | (grp < 3);
@@ -452,11 +465,12 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
       const int a = c27 / 9, b = (c27 / 3) % 3, c = c27 % 3;
       const bool in = kin & (wx[a] != 0.0) & (wy[b] != 0.0) & (wz[c] != 0.0);
       const double w = (wx[a] * wy[b]) * wz[c];
-      const int m = ((2 * ix + a - 1) * gf.NY + (2 * iy + b - 1)) * gf.NZ + (2 * iz + c - 1);
+      const int m =
+          in ? ((2 * ix + a - 1) * gf.NY + (2 * iy + b - 1)) * gf.NZ + (2 * iz + c - 1) : 0;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const int f = in ? mapf(3 * m + q) : -1;
-        const double v = rf[f >= 0 ? f : 0];
+        const int f = in ? map_idx<OFFF>(mapf, 3 * m + q) : -1;
+        const double v = rf[f >= 0 ? (unsigned)f : 0u];
         if (f >= 0) acc[q] += w * v;
       }
     }
@@ -484,10 +498,22 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
   }
 }
 
+#define GF_TRANSFER(KERNEL, GRID, ...)                                                     \
+  do {                                                                                     \
+    const bool of_ = mapf.off >= 0, oc_ = mapc.off >= 0;                                   \
+    if (of_ && oc_)                                                                        \
+      hipLaunchKernelGGL((KERNEL<true, true>), GRID, dim3(kBlock), 0, st, __VA_ARGS__);    \
+    else if (of_)                                                                          \
+      hipLaunchKernelGGL((KERNEL<true, false>), GRID, dim3(kBlock), 0, st, __VA_ARGS__);   \
+    else if (oc_)                                                                          \
+      hipLaunchKernelGGL((KERNEL<false, true>), GRID, dim3(kBlock), 0, st, __VA_ARGS__);   \
+    else                                                                                   \
+      hipLaunchKernelGGL((KERNEL<false, false>), GRID, dim3(kBlock), 0, st, __VA_ARGS__);  \
+  } while (0)
+
 int prolong(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& mapc,
             const double* ec, double* yf, const int* gate, hipStream_t st) {
-  hipLaunchKernelGGL(k_prolong, dim3(cdiv(gf.n_nodes, kBlock)), dim3(kBlock), 0, st, gf, mapf,
-                     gc, mapc, ec, yf, gate);
+  GF_TRANSFER(k_prolong, dim3(cdiv(gf.n_nodes, kBlock)), gf, mapf, gc, mapc, ec, yf, gate);
   GF_CHECK_LAUNCH();
   return 0;
 }
@@ -496,11 +522,11 @@ int restrict_(const Grid& gf, const DofMap& mapf, const Grid& gc, const DofMap& 
               const double* rf, double* yc, const int* gate, hipStream_t st,
               const ZeroStart* zs) {
   const ZeroStart z = zs ? *zs : ZeroStart{};
-  hipLaunchKernelGGL(k_restrict, dim3(cdiv(gc.n_nodes, 64)), dim3(kBlock), 0, st, gf, mapf,
-                     gc, mapc, rf, yc, z, gate);
+  GF_TRANSFER(k_restrict, dim3(cdiv(gc.n_nodes, 64)), gf, mapf, gc, mapc, rf, yc, z, gate);
   GF_CHECK_LAUNCH();
   return 0;
 }
+#undef GF_TRANSFER
 
 // ---------------------------------------------------------------------------
 // Galerkin product K_c = R (K_f P), in two stages with the association of the

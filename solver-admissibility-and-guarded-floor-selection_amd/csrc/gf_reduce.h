@@ -29,6 +29,27 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return r;
 }
 
+// NP sums over a 256-thread block at once, each by block_sum's tree (the
+// same bits as NP block_sum calls), with one pair of barriers for all of
+// them; lds holds 4 * NP doubles.  Results valid in thread 0 (v[p]).
+template <int NP>
+__device__ __forceinline__ void block_sum_n(double (&v)[NP], double* lds) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) v[p] = wave_sum(v[p]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) lds[4 * p + wid] = v[p];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      v[p] = (lds[4 * p] + lds[4 * p + 1]) + (lds[4 * p + 2] + lds[4 * p + 3]);
+  }
+  __syncthreads();
+}
+
 // One thread's share of sum_i x_i y_i over a grid-stride loop: four
 // strides' loads issue before the first is used, and the sum is taken in the
 // same order as the plain stride loop (identical bits).
@@ -65,11 +86,12 @@ __device__ __forceinline__ double fold_thread_first(int nb, const double* partia
 template <int NP>
 __device__ __forceinline__ bool finish_reduce(const double (&v)[NP], double* partials,
                                               int* ticket, double* out, int post) {
-  __shared__ double lds[4];
+  __shared__ double lds[4 * NP];
   __shared__ bool last;
   double bs[NP];
 #pragma unroll
-  for (int p = 0; p < NP; ++p) bs[p] = block_sum(v[p], lds);
+  for (int p = 0; p < NP; ++p) bs[p] = v[p];
+  block_sum_n<NP>(bs, lds);
   // partials and ticket live in uncached device memory, so a partial is
   // visible to every XCD once its store is acknowledged.  Waiting for that
   // acknowledgement (vmcnt 0) before the ticket increment is the part of an
@@ -87,17 +109,18 @@ __device__ __forceinline__ bool finish_reduce(const double (&v)[NP], double* par
   __syncthreads();
   if (!last) return false;
   const int nb = gridDim.x;
+  double a[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
-    double a = 0.0;
+    a[p] = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x)
-      a += __hip_atomic_load(&partials[p * kRedBlocks + i], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    double s = block_sum(a, lds);
-    if (threadIdx.x == 0) {
-      if (post == 1) s = sqrt(s);
-      out[p] = s;
-    }
+      a[p] += __hip_atomic_load(&partials[p * kRedBlocks + i], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum_n<NP>(a, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) out[p] = post == 1 ? sqrt(a[p]) : a[p];
   }
   if (threadIdx.x == 0) __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
