@@ -1143,7 +1143,20 @@ __global__ __launch_bounds__(kBlock) void k_dense_gemv(int n, const double* __re
   if (row >= n) return;
   const double* r = K + (long)row * n;
   double acc = 0.0;
-  for (int c = lane; c < n; c += 64) acc += r[c] * b[c];
+  // four columns' loads in flight before the first product, the products
+  // added in ascending column order (the plain stride loop's order)
+  int c = lane;
+  for (; c + 192 < n; c += 256) {
+    double rv[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      rv[u] = r[c + 64 * u];
+      bv[u] = b[c + 64 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += rv[u] * bv[u];
+  }
+  for (; c < n; c += 64) acc += r[c] * b[c];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
   if (lane == 0) x[row] = acc;

@@ -315,16 +315,28 @@ int Ctx::nodeblock_alloc() {
 int Ctx::autotune(hipStream_t st, float* us_out) {
   GF_DEVICE_LOCK();
   const long ne = (long)g.nx * g.ny * g.nz;
-  double *E = nullptr, *u = nullptr, *y = nullptr;
-  GF_TRY(dalloc(&E, ne));
-  GF_TRY(dalloc(&u, n_free));
-  GF_TRY(dalloc(&y, n_free));
+  // timing buffers and events, released on every return path
+  struct Scratch {
+    double *E = nullptr, *u = nullptr, *y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Scratch() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      dfree(E);
+      dfree(u);
+      dfree(y);
+    }
+  } sc;
+  GF_TRY(dalloc(&sc.E, ne));
+  GF_TRY(dalloc(&sc.u, n_free));
+  GF_TRY(dalloc(&sc.y, n_free));
+  double *E = sc.E, *u = sc.u, *y = sc.y;
   hipLaunchKernelGGL(k_fill, dim3(256), dim3(kBlock), 0, st, ne, E, 1.0, (const int*)nullptr);
   hipLaunchKernelGGL(k_fill, dim3(256), dim3(kBlock), 0, st, (long)n_free, u, 1.0,
                      (const int*)nullptr);
-  hipEvent_t e0, e1;
-  GF_HIP(hipEventCreate(&e0));
-  GF_HIP(hipEventCreate(&e1));
+  GF_HIP(hipEventCreate(&sc.e0));
+  GF_HIP(hipEventCreate(&sc.e1));
+  hipEvent_t e0 = sc.e0, e1 = sc.e1;
   // the shapes are timed in three interleaved rounds and each keeps its best
   // round, so a burst of other work on the device (concurrent instances set
   // up at the same time) does not decide the choice
@@ -406,11 +418,6 @@ int Ctx::autotune(hipStream_t st, float* us_out) {
       }
     g.shape32 = best32;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  dfree(E);
-  dfree(u);
-  dfree(y);
   return 0;
 }
 

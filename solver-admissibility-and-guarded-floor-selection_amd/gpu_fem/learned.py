@@ -716,10 +716,12 @@ def _evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load
         return sum(int(r["solve_converged"]) for r in rows)
 
     out = {}
+    runs = {}
     for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", "auto")):
         got, wall = run_learned(model, states, workers=workers, load=load, densities=dens,
                                 progress=progress, chunk=chunk, confidence_threshold=gate,
                                 **kw, **precision_stack("fp32"))
+        runs[name] = got
         c = compare(plain, got)
         c.update(value=acc(got) / wall, wall_s=wall,
                  speedup_vs_plain_fp64_concurrent=(acc(got) / wall) / (acc(plain) / w_plain),
@@ -731,6 +733,24 @@ def _evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load
                                                        [got[i] for i in idx])
                          for d, idx in groups.items()}
         out[name] = c
+    # one record per state: the plain floor, the network's proposal and
+    # confidence, the gate of its grid and what each learned run accepted
+    # (the gated outcome for another gate follows from these: a state at or
+    # above the gate takes the ungated run's result, one below it the plain
+    # policy's)
+    ug, ga = runs["fp32_ungated"], runs["fp32_gated"]
+    out["rows"] = [{
+        "dims": "x".join(map(str, s["dims"])), "kind": s["kind"], "seed": s["seed"],
+        "plain_rho_min": p["recommended_rho_min"], "plain_trigger": p["trigger"],
+        "plain_iters": len(p["probe_history"]) - 1
+        + sum(len(h) - 1 for _, h in p["solve_history"]),
+        "predicted_class": u["predicted_class"], "confidence": u["predicted_confidence"],
+        "gate": model.gate_for(s["dims"]),
+        "ungated_rho_min": u["recommended_rho_min"], "ungated_fallback": u["guard_fallback"],
+        "ungated_iters": u["total_iters"],
+        "gated_rho_min": g["recommended_rho_min"], "gated_path": g.get("gate", ""),
+        "gated_iters": g["total_iters"],
+    } for s, p, u, g in zip(states, plain, ug, ga)]
     return out
 
 
@@ -741,10 +761,14 @@ def main(argv=None):
     python -m gpu_fem.learned train --labels a.csv,b.csv --out floor_cnn.npz
     python -m gpu_fem.learned evaluate --model floor_cnn.npz --grids 80x40x20 --states 96 \
         --seed 11 --out eval.json
+    python -m gpu_fem.learned calibrate --model floor_cnn.npz --labels c.csv --target 0.995 \
+        --out floor_cnn_cal.npz
 
     ``train`` holds out every seventh state of each grid to calibrate the
     per-grid confidence gates, trains on the rest (Adam, BF16 kernels) and
-    saves the parameters with the training grids and the gates."""
+    saves the parameters with the training grids and the gates.  ``calibrate``
+    sets the gates of a trained network afresh from labelled states it was
+    not trained on (grids absent from those labels keep their gates)."""
     import argparse
     import sys
 
@@ -767,6 +791,11 @@ def main(argv=None):
     ev.add_argument("--workers", type=int, default=12)
     ev.add_argument("--penal", type=float, default=3.0)
     ev.add_argument("--out", required=True)
+    ca = sub.add_parser("calibrate")
+    ca.add_argument("--model", required=True)
+    ca.add_argument("--labels", required=True)
+    ca.add_argument("--target", type=float, default=0.99)
+    ca.add_argument("--out", required=True)
     tr = sub.add_parser("train")
     tr.add_argument("--labels", required=True)
     tr.add_argument("--epochs", type=int, default=60)
@@ -807,6 +836,14 @@ def main(argv=None):
         rows += r_
     say(f"pooling {len(rows)} labelled states")
     attach_pooled(states, rows)
+    if a.cmd == "calibrate":
+        model = FloorCNN.load(a.model)
+        gates = calibrate_gates(model, rows, a.target)
+        say(f"gates at {a.target}: {gates} (before: {model.gates})")
+        model.gates.update(gates)
+        model.save(a.out)
+        say(f"wrote {a.out}")
+        return
     by_grid = {}
     for i, r in enumerate(rows):
         by_grid.setdefault(tuple(r["dims"]), []).append(i)
