@@ -497,15 +497,14 @@ class LearnedFloorPolicy(GuardedFloorPolicy):
         self.floors = tuple(float(f) for f in floors)
         self.confidence_threshold = float(confidence_threshold)
 
+    # a session's first state may be solved at its proposed floor first; the
+    # session sets this instance up with the probe's moduli before it, so the
+    # cached spectral estimate is the plain session's and a state sent to the
+    # probe is solved exactly as the plain policy solves it (same stack)
+    primes_first_state = True
+
     def _gated(self, confidence, out_of_range=False) -> bool:
         return bool(out_of_range) or float(confidence) < self.confidence_threshold
-
-    def first_moduli(self, state: dict) -> np.ndarray:
-        ex = state.get("extra", {})
-        if self._gated(ex.get("predicted_confidence", 1.0), ex.get("out_of_range", False)):
-            return super().first_moduli(state)
-        k = ex.get("predicted_class", 0)
-        return self.moduli(self.floors[k], state.get("solid"), state.get("rho_field"))
 
     def run_state(self, solid=None, *, rho_field=None, seed=None, probability=None,
                   keep_x=False, predicted_class=0, predicted_r50=float("nan"),
@@ -672,10 +671,11 @@ def calibrate_gates(model: "FloorCNN", rows, target: float = 0.99) -> dict:
 
 
 def evaluate(model, states, *, workers: int = 12, load: str = "edge", progress=None,
-             chunk: int = 16, **kw) -> dict:
+             chunk: int = 16, precision: str = "fp32", **kw) -> dict:
     """Fresh-state evaluation: the plain FP64 policy (the reference floor),
-    then the learned policy with FP32 multigrid ungated and with the
-    network's per-grid confidence gates, all on ``workers`` concurrent
+    then the learned policy with ``precision`` multigrid ("fp32" or "fp64")
+    ungated and with the network's per-grid confidence gates, all on
+    ``workers`` concurrent
     instances.  Returns the comparison of each learned run with the plain
     one, over all states and per grid, with accepted solves/s.  ``model``
     may be a dict of named networks evaluated on the same plain run; the
@@ -701,15 +701,15 @@ def evaluate(model, states, *, workers: int = 12, load: str = "edge", progress=N
             if progress is not None:
                 progress(f"evaluating {mname}")
             out[mname] = _evaluate_learned(m, states, dens, plain, w_plain, groups, workers,
-                                           load, progress, chunk, kw)
+                                           load, progress, chunk, precision, kw)
         return out
     out.update(_evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load,
-                                 progress, chunk, kw))
+                                 progress, chunk, precision, kw))
     return out
 
 
 def _evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load, progress,
-                      chunk, kw) -> dict:
+                      chunk, precision, kw) -> dict:
     from .batch import precision_stack
 
     def acc(rows):
@@ -717,10 +717,10 @@ def _evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load
 
     out = {}
     runs = {}
-    for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", "auto")):
+    for name, gate in ((f"{precision}_ungated", 0.0), (f"{precision}_gated", "auto")):
         got, wall = run_learned(model, states, workers=workers, load=load, densities=dens,
                                 progress=progress, chunk=chunk, confidence_threshold=gate,
-                                **kw, **precision_stack("fp32"))
+                                **kw, **precision_stack(precision))
         runs[name] = got
         c = compare(plain, got)
         c.update(value=acc(got) / wall, wall_s=wall,
@@ -738,7 +738,7 @@ def _evaluate_learned(model, states, dens, plain, w_plain, groups, workers, load
     # (the gated outcome for another gate follows from these: a state at or
     # above the gate takes the ungated run's result, one below it the plain
     # policy's)
-    ug, ga = runs["fp32_ungated"], runs["fp32_gated"]
+    ug, ga = runs[f"{precision}_ungated"], runs[f"{precision}_gated"]
     out["rows"] = [{
         "dims": "x".join(map(str, s["dims"])), "kind": s["kind"], "seed": s["seed"],
         "plain_rho_min": p["recommended_rho_min"], "plain_trigger": p["trigger"],
@@ -790,6 +790,7 @@ def main(argv=None):
     ev.add_argument("--seed", type=int, required=True)
     ev.add_argument("--workers", type=int, default=12)
     ev.add_argument("--penal", type=float, default=3.0)
+    ev.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ev.add_argument("--out", required=True)
     ca = sub.add_parser("calibrate")
     ca.add_argument("--model", required=True)
@@ -823,8 +824,9 @@ def main(argv=None):
         model = (FloorCNN.load(paths[0]) if len(paths) == 1
                  else {m: FloorCNN.load(m) for m in paths})
         say(f"evaluating {a.model} on {len(states)} fresh states on {grids}")
-        res = evaluate(model, states, workers=a.workers, progress=say, penal=a.penal)
-        res.update(model=a.model, seed=a.seed)
+        res = evaluate(model, states, workers=a.workers, progress=say, penal=a.penal,
+                       precision=a.precision)
+        res.update(model=a.model, seed=a.seed, precision=a.precision)
         with open(a.out, "w") as f:
             json.dump(res, f, indent=1)
         say(f"wrote {a.out}")

@@ -173,10 +173,16 @@ class GuardedFloorPolicy:
         return cls(spec, preset_name=f"cantilever_{dims[0]}x{dims[1]}x{dims[2]}_{load}", **kw)
 
     # -------------------------------------------------------------------
+    # True for a policy whose first set-up of a state may be at another
+    # floor than the probe's (the learned policy's proposals): a session then
+    # primes its first instance with the probe's moduli before that state
+    primes_first_state = False
+
     def first_moduli(self, state: dict) -> np.ndarray:
-        """Moduli of this policy's first set-up on ``state`` (the probe at the
-        baseline floor); a session primes its cached spectral estimate with
-        them."""
+        """Moduli of the plain policy's first set-up on ``state`` (the probe at
+        the baseline floor).  A session sets up every instance with them
+        first, so its cached spectral estimate is the plain session's
+        whichever policy it runs."""
         return self.moduli(self.baseline_rho_min, state.get("solid"), state.get("rho_field"))
 
     def moduli(self, rho_min: float, solid=None, rho_field=None) -> np.ndarray:
@@ -402,10 +408,14 @@ class PolicySession:
         start = 0
         if not self._started and states:
             st0 = states[0]
+            E0 = torch.as_tensor(self.runners[0].first_moduli(st0), device="cuda")
+            if self.runners[0].primes_first_state:
+                with torch.cuda.stream(self.streams[0]):
+                    self.runners[0].gmg.setup(E0)
+                    _sync()
             rows[0] = self._run_one(0, st0)
             if on_done is not None:
                 on_done(0)
-            E0 = torch.as_tensor(self.runners[0].first_moduli(st0), device="cuda")
             for k in range(1, self.workers):
                 with torch.cuda.stream(self.streams[k]):
                     self.runners[k].gmg.setup(E0)

@@ -259,3 +259,39 @@ def test_evaluate_reports_agreement_against_the_plain_policy():
         if small is not None:
             assert small["out_of_range_rate"] == 1.0 and small["floor_agreement"] == 1.0
         assert c["max_accepted_true_rel_residual"] <= 1e-6
+    # one record per state, in state order
+    assert [r["seed"] for r in res["rows"]] == [s["seed"] for s in states]
+    for r in res["rows"]:
+        if r["gated_path"] == "proposal":
+            assert r["confidence"] >= r["gate"]
+
+
+def test_probed_states_are_solved_as_the_plain_policy_solves_them():
+    """With the plain policy's FP64 stack, a state that the confidence gate
+    sends to the probe gives the plain policy's row bit for bit (trigger,
+    floor, probe and solve histories), also when the session's first state
+    is solved at a proposed floor: the session sets that instance up with
+    the probe's moduli first, so its cached spectral estimate is the plain
+    session's."""
+    from gpu_fem import learned as L
+
+    model = L.FloorCNN.load()
+    states = L.sample_states(8, 43, [(40, 20, 20)])
+    _, _, conf = L.predict_states(model, states)
+    conf = np.asarray(conf, dtype=np.float64)
+    order = np.argsort(-conf, kind="stable")  # the most confident state first
+    states = [states[i] for i in order]
+    conf = conf[order]
+    tau = float(np.median(conf))
+    assert conf[0] > tau
+    plain = L.label_states(states, workers=2)
+    learned, _ = L.run_learned(model, states, workers=2, confidence_threshold=tau)
+    assert learned[0]["gate"] == "proposal"
+    probed = [i for i, r in enumerate(learned) if r["gate"] == "probe"]
+    assert len(probed) >= 3
+    for i in probed:
+        p, q = plain[i], learned[i]
+        assert q["trigger"] == p["trigger"]
+        assert q["recommended_rho_min"] == p["recommended_rho_min"]
+        assert q["probe_history"] == p["probe_history"]
+        assert q["solve_history"] == p["solve_history"]
