@@ -105,7 +105,8 @@ def parse():
                     help="fix the filtered-sigmoid volume fraction of the evaluation states")
     ap.add_argument("--eval-workers", type=int, default=0,
                     help="concurrent instances of the evaluation (0: --workers)")
-    ap.add_argument("--learned-variants", default="fp64,fp32,fp32_gated")
+    ap.add_argument("--learned-variants", default="fp64,fp32,fp32_gated",
+                    help="comma list of fp64, fp32, fp32_gated, fp64_gated")
     ap.add_argument("--eval-warmup", type=int, default=1, choices=[0, 1])
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--labels", default="",
@@ -289,12 +290,14 @@ def reference_parametrisation(args):
 
 def learned_on_states(args, spec, kw, dims, warm, timed, plain_rows, wall_plain, wall_seq):
     """The learned policy with the shipped network (gpu_fem/data) on the
-    timed states, FP32 multigrid, --workers concurrent instances: ungated
-    (every in-range state solved at the proposed floor) and gated (states
-    below the network's calibrated confidence go through the plain probe).
-    Each runs as its own session after a warm-up on the same warm-up states
-    as the plain line; the pooling and the network's forward pass are
-    inside the timed region."""
+    timed states, --workers concurrent instances: with FP32 multigrid
+    ungated (every in-range state solved at the proposed floor) and gated
+    (states below the network's calibrated confidence go through the plain
+    probe), and gated with the plain policy's FP64 stack (a probed state is
+    then solved exactly as the plain line solves it).  Each runs as its own
+    session after a warm-up on the same warm-up states as the plain line;
+    the pooling and the network's forward pass are inside the timed
+    region."""
     import numpy as np
     import torch
 
@@ -313,10 +316,12 @@ def learned_on_states(args, spec, kw, dims, warm, timed, plain_rows, wall_plain,
         feat = torch.as_tensor(np.stack([L.grid_features(dims)] * len(sts)), device="cuda")
         return model.predict_proba(X, feat)
 
-    for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", model.gate_for(dims))):
+    tau = model.gate_for(dims)
+    for name, gate, prec in (("fp32_ungated", 0.0, "fp32"), ("fp32_gated", tau, "fp32"),
+                             ("fp64_gated", tau, "fp64")):
         progress(f"learned ({name}, gate {gate:.3f}): warm-up and {len(timed)} timed states")
         session = gpol.PolicySession(spec, workers=args.workers, runner_cls=L.LearnedFloorPolicy,
-                                     confidence_threshold=gate, **kw, **precision_stack("fp32"))
+                                     confidence_threshold=gate, **kw, **precision_stack(prec))
 
         def attach(sts):
             cls, r50, conf = predict(sts)
@@ -781,7 +786,8 @@ def learned_form(args):
     progress(f"learned: plain FP64 policy, {workers} concurrent instances")
     plain, w_plain = timed(lambda: L.label_states(ev, workers=workers, load=args.load,
                                                   densities=dens, progress=progress, **pk))
-    variants = {"fp64": ("fp64", 0.0), "fp32": ("fp32", 0.0), "fp32_gated": ("fp32", tau)}
+    variants = {"fp64": ("fp64", 0.0), "fp32": ("fp32", 0.0), "fp32_gated": ("fp32", tau),
+                "fp64_gated": ("fp64", tau)}
     forms = {}
     for name in (v for v in args.learned_variants.split(",") if v):
         prec, gate = variants[name]
