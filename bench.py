@@ -6,7 +6,8 @@ headline number, recorded_policy_time_s).  Workload (default): cantilever
 80x40x20 (64,000 hex8 elements, 206,640 free DOFs), E0=1, nu=0.3, SIMP p=3,
 x=0 face clamped, unit downward load split over the x=L, y=0 edge nodes;
 Bernoulli densities p in {0.2, 0.35, 0.5} over seeds 0..31 (taken in
-seed-major order), ladder {1e-12 probe, 1e-3, 1e-2}, probe 100 / r50 at 50,
+seed-major order: 96 states, the first 12 a warm-up and the other 84
+timed by default), ladder {1e-12 probe, 1e-3, 1e-2}, probe 100 / r50 at 50,
 thresholds 1e-2 / 1e-4 / 0.6, FGMRES restart=maxiter=300, tol 1e-6.
 
 Prints ONE JSON line (the driver contract) with a roofline object for the
@@ -59,7 +60,8 @@ def progress(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--steps", type=int, default=84,
+                    help="timed states (default: every state after the warm-up)")
     ap.add_argument("--warmup", type=int, default=12)
     ap.add_argument("--grid", default="80x40x20")
     ap.add_argument("--penal", type=float, default=3.0)
