@@ -138,27 +138,30 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
                                                           double* __restrict__ w, double* H,
                                                           double* ring, int* flags, double* err,
                                                           const int* gate, GivensArgs gv) {
-  if (gate && *gate == 0) return;  // uniform: written by an earlier kernel
+  const bool open = gate_open(gate);  // tested before the first exchange
   __shared__ double lds[4];
   __shared__ double hsh;
   __shared__ unsigned f0s;
   const int G = gridDim.x, b = blockIdx.x;
-  // Every complete sweep advances every block's ring position by j+2 (mod 3),
-  // so all positions are equal when a sweep starts.
-  if (threadIdx.x == 0)
-    f0s = (unsigned)__hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const int r0 = (int)(f0s % 3u);
   const long stride = (long)G * kSweepBlock;
   const long base = (long)b * kSweepBlock + threadIdx.x;
   double wr[PER], vr[PER], vn[PER];
   bool broken = false;  // wave 0 only
+  // w and V_0 are requested before the ring position, so the two load
+  // latencies overlap at the barrier below
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const long i = base + k * stride;
     wr[k] = i < n ? w[i] : 0.0;
     vr[k] = i < n ? V[i] : 0.0;
   }
+  // Every complete sweep advances every block's ring position by j+2 (mod 3),
+  // so all positions are equal when a sweep starts.
+  if (threadIdx.x == 0)
+    f0s = (unsigned)__hip_atomic_load(&flags[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (!open) return;  // uniform over the grid: no block enters the exchange
+  const int r0 = (int)(f0s % 3u);
   for (int s = 0; s <= j + 1; ++s) {
     double a = 0.0;
     if (s <= j) {
@@ -430,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_ls_fused(long n, const double* __res
                                                      double* stat, double max_scale, double ratio,
                                                      double* parts, unsigned long long* cnt,
                                                      double* err, const int* gate) {
-  if (gate && *gate == 0) return;  // uniform: written by an earlier kernel
+  const bool open = gate_open(gate);  // tested before the first exchange
   __shared__ double lds[16];
   __shared__ double sst[8];
   __shared__ unsigned long long base_s;
@@ -468,6 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_ls_fused(long n, const double* __res
     }
   }
   block_sum_n<4>(a, lds);
+  if (!open) return;  // uniform over the grid: no block enters the exchange
   const unsigned long long base = base_s;  // written by thread 0 before block_sum_n's barriers
   if (threadIdx.x == 0) {
 #pragma unroll

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64 * NW) void k_stencil_nodes(Grid g, DofMap map,
                                                            const int* gate) {
   constexpr int S = 64 / NPB;                   // sub-slots per wave and round
   constexpr int R = (27 + NW * S - 1) / (NW * S);  // slot rounds per wave
-  if (gate && *gate == 0) return;
+  const bool open = gate_open(gate);  // tested after the slot loads
   __shared__ T tk[27][3][NPB];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64 * NW) void k_stencil_nodes(Grid g, DofMap map,
     }
   }
   __syncthreads();
-  if (!folder || !valid) return;
+  if (!open || !folder || !valid) return;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     if (o[r] < 0) continue;
@@ -379,7 +379,7 @@ template <bool OFFF, bool OFFC>
 __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid gc, DofMap mapc,
                                                     const double* __restrict__ ec,
                                                     double* __restrict__ yf, const int* gate) {
-  if (gate && *gate == 0) return;
+  const bool open = gate_open(gate);  // tested after the parent loads
   const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= gf.n_nodes) return;
   const int o0 = map_idx<OFFF>(mapf, 3 * n), o1 = map_idx<OFFF>(mapf, 3 * n + 1),
@@ -413,6 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Grid gf, DofMap mapf, Grid g
           if (f >= 0) acc[q] += w * v;
         }
       }
+  if (!open) return;
   if (o0 >= 0) yf[o0] = acc[0];
   if (o1 >= 0) yf[o1] = acc[1];
   if (o2 >= 0) yf[o2] = acc[2];
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
                                                      const double* __restrict__ rf,
                                                      double* __restrict__ yc, ZeroStart zs,
                                                      const int* gate) {
-  if (gate && *gate == 0) return;
+  const bool open = gate_open(gate);  // tested after the child loads
   __shared__ double part[3][3][64];
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
   const int n = xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid gf, DofMap mapf, Grid 
     for (int q = 0; q < 3; ++q) part[grp - 1][q][lane] = acc[q];
   }
   __syncthreads();
-  if (grp != 0 || !valid) return;
+  if (!open || grp != 0 || !valid) return;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const double v = ((acc[q] + part[0][q][lane]) + part[1][q][lane]) + part[2][q][lane];
@@ -1137,7 +1138,7 @@ int dense_inverse_from_chol(int n, const double* L, double* Y, double* /*Yt*/, d
 __global__ __launch_bounds__(kBlock) void k_dense_gemv(int n, const double* __restrict__ K,
                                                        const double* __restrict__ b,
                                                        double* __restrict__ x, const int* gate) {
-  if (gate && *gate == 0) return;
+  const bool open = gate_open(gate);  // tested after the row's loads
   const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
@@ -1159,7 +1160,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_gemv(int n, const double* __re
   for (; c < n; c += 64) acc += r[c] * b[c];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-  if (lane == 0) x[row] = acc;
+  if (lane == 0 && open) x[row] = acc;
 }
 
 int dense_gemv(int n, const double* K, const double* b, double* x, const int* gate,

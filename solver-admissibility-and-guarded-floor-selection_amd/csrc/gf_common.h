@@ -165,6 +165,35 @@ __device__ __forceinline__ T map_load(const DofMap& m, const T* __restrict__ u, 
   return ok ? v : T(0);
 }
 
+// Launch gate of a kernel in a device-controlled sequence: a flag (nullable)
+// written by an earlier kernel on the stream, so its value is uniform over
+// the grid and visible from the kernel's start like any input.  A kernel
+// requests it at its top and tests it just before its first global store
+// (or grid-wide exchange), so the flag's load latency passes under the
+// kernel's own loads; a closed gate then costs only loads of the launch's
+// valid buffers.  The flag is read by a vector load (a relaxed atomic load
+// is never a scalar-cache load): vector loads complete in order, so the
+// test waits for the flag only, while a scalar load would be waited for at
+// the first use of any later scalar load (the kernel's arguments).
+// A kernel without a gate reads this always-open flag instead, so the load
+// is unconditional (a branch around it would be joined, and so waited for,
+// at once).
+static __device__ const int kGateAlwaysOpen = 1;
+
+__device__ __forceinline__ bool gate_open(const int* gate) {
+  // a global-memory load: a flat load would also count as a scalar-memory
+  // access and be waited for with the kernel's arguments
+  using GInt = const __attribute__((address_space(1))) int;
+  GInt* p = (GInt*)(gate ? gate : &kGateAlwaysOpen);
+  // a zero offset held in a vector register: the flag's address then looks
+  // lane-varying to the compiler, which keeps the flag in a vector register
+  // until its test (a uniform flag would be moved to a scalar register, and
+  // waited for, right after its load)
+  int zero;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+  return __hip_atomic_load(p + zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+}
+
 // Device scalar slots (FP64) used by device-resident control flow.
 struct Scalars {
   double* s;   // base pointer of a device array of doubles

@@ -647,7 +647,7 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
   constexpr int WY = BY + 2;
   constexpr int EY = BY + 1;
   static_assert(AS == 1 || (AS == 2 && NB % 64 == 0), "lane groups must be whole waves");
-  if (gate && *gate == 0) return;
+  const bool open = gate_open(gate);  // tested once the window is staged
   __shared__ T us[(BX + 2) * WY * RP];
   __shared__ T es[(BX + 1) * EY * EZP];
   __shared__ A q1[AS == 2 ? 3 * NB : 1];
@@ -656,6 +656,7 @@ __global__ __launch_bounds__(AS* BX* BY* BZ) void k_fine_matvec_box(Grid g, DofM
   const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, tx = tile / (tiles_z * tiles_y);
   const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
   stage_box<T, OFF, BX, BY, BZ, RP, EZP, AS * NB>(g, map, E, u, us, es, x0, y0, z0);
+  if (!open) return;  // uniform over the block, after stage_box's barrier
   const int grp = AS == 2 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / NB) : 0;
   const int t = threadIdx.x - grp * NB;
   const int lz = t % BZ, ly = (t / BZ) % BY, lx = t / (BZ * BY);
@@ -878,16 +879,18 @@ __global__ __launch_bounds__(kBlock) void k_zero_start(long n, const T* __restri
                                                        const T* __restrict__ dinv, T a,
                                                        T* __restrict__ dv, T* __restrict__ x,
                                                        const int* gate) {
-  if (gate && *gate == 0) return;
+  const bool open = gate_open(gate);  // tested after the first element's loads
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
+    const T di = dinv[i], bi = b[i];
+    if (!open) return;
     // explicitly rounded, as in the restriction's fused form (ZeroStart)
     if (dv) {
-      const T z = rn_mul(dinv[i], b[i]);
+      const T z = rn_mul(di, bi);
       dv[i] = z;
       x[i] = rn_add(T(0), rn_mul(a, z));
     } else {
-      x[i] = rn_add(T(0), rn_mul(rn_mul(a, dinv[i]), b[i]));
+      x[i] = rn_add(T(0), rn_mul(rn_mul(a, di), bi));
     }
   }
 }
@@ -1040,18 +1043,23 @@ __global__ __launch_bounds__(kBlock) void k_nodeblock_apply(Grid g, DofMap map,
                                                             const double* __restrict__ r,
                                                             double* __restrict__ z,
                                                             const int* gate) {
-  if (gate && *gate == 0) return;
+  const bool open = gate_open(gate);  // tested after the node's loads
   const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= g.n_nodes) return;
   const int o0 = map(3 * n), o1 = map(3 * n + 1), o2 = map(3 * n + 2);
   if ((o0 >= 0) & (o1 >= 0) & (o2 >= 0)) {
     const double* I = inv9 + 9L * n;
     const double r0 = r[o0], r1 = r[o1], r2 = r[o2];
-    z[o0] = I[0] * r0 + I[1] * r1 + I[2] * r2;
-    z[o1] = I[3] * r0 + I[4] * r1 + I[5] * r2;
-    z[o2] = I[6] * r0 + I[7] * r1 + I[8] * r2;
+    const double z0 = I[0] * r0 + I[1] * r1 + I[2] * r2;
+    const double z1 = I[3] * r0 + I[4] * r1 + I[5] * r2;
+    const double z2 = I[6] * r0 + I[7] * r1 + I[8] * r2;
+    if (!open) return;
+    z[o0] = z0;
+    z[o1] = z1;
+    z[o2] = z2;
     return;
   }
+  if (!open) return;
   const int o[3] = {o0, o1, o2};
 #pragma unroll
   for (int q = 0; q < 3; ++q)
