@@ -1132,9 +1132,9 @@ def main():
             "unit": "GB/s",
             "frac": sten["GB_s"] / HBM_PEAK_GBS,
             "traffic": STENCIL_TRAFFIC_BYTES,
-            "note": ("the policy's largest kernel family (k_stencil_nodes 26.1 % of the "
-                     "summed kernel time, the fine operator 23.5 %, the Gram-Schmidt sweep "
-                     "21.4 %, profiles/policy_state_80x40x20_fp64_kernel_stats.md), with an "
+            "note": ("the policy's largest kernel family (k_stencil_nodes 27.2 % of the "
+                     "summed kernel time, the fine operator 23.9 %, the Gram-Schmidt sweep "
+                     "21.7 %, profiles/policy_state_80x40x20_fp64_kernel_stats.md), with an "
                      "HBM roof: 0.24 flop/B; timed round-robin over set-up hierarchies whose "
                      "stencils together exceed the 256 MB Infinity Cache; the fine operator's "
                      "FP64-compute roofline is reported beside it"),
