@@ -79,7 +79,7 @@ def parse():
                          "launch, 0 one launch per projection")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="HIP hardware queues for this process (GPU_MAX_HW_QUEUES, <= 32; "
-                         "0 keeps the runtime default)")
+                         "0 keeps the runtime default; --form precision: 1)")
     ap.add_argument("--states-csv", default="")
     ap.add_argument("--form", default="policy",
                     choices=["policy", "batched", "precision", "learned", "coarsest"],
@@ -924,6 +924,11 @@ def learned_eval(args, model, pk):
 
 def main():
     args = parse()
+    if args.form == "precision" and not args.hw_queues:
+        # one solver instance at a time: with one hardware queue the
+        # co-residency rule admits the 1M-element Gram-Schmidt sweep (755
+        # blocks) as one persistent launch per Arnoldi step
+        args.hw_queues = 1
     if args.hw_queues:
         if not 1 <= args.hw_queues <= 32:
             raise SystemExit("--hw-queues must be in [1, 32]")

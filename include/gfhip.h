@@ -163,6 +163,14 @@ int32_t gf_set_coarse_form(int32_t form);
  * its grid fits the co-residency rule (default), 0 = three launches.  Both
  * use the same reduction trees: identical results. */
 int32_t gf_set_ls_form(int32_t form);
+/* Process-wide form of the persistent Gram-Schmidt sweep (the modified
+ * Gram-Schmidt of multigrid_v4.py:474-482 in one launch per Arnoldi step; no
+ * reference counterpart as a switch): 1 = the sweep that prefetches the next
+ * basis vector where its grid fits the co-residency rule, else the one
+ * without that prefetch (fewer registers: a larger grid fits), else one
+ * launch per projection (default); 2 = the sweep without the prefetch
+ * wherever it fits.  The sweeps give identical results. */
+int32_t gf_set_sweep_form(int32_t form);
 /* y_free = K_ff(E) u_free. E: n_elem moduli; u,y: n_free (device). */
 int32_t gf_matvec_f64(gf_ctx* ctx, const double* E, const double* u, double* y, void* stream);
 int32_t gf_matvec_f32(gf_ctx* ctx, const float* E, const float* u, float* y, void* stream);

@@ -148,6 +148,15 @@ int32_t gf_set_ls_form(int32_t form) {
   return 0;
 }
 
+int32_t gf_set_sweep_form(int32_t form) {
+  if (form != 1 && form != 2) {
+    set_error("gf_set_sweep_form: form must be 1 or 2");
+    return -2;
+  }
+  set_sweep_form(form);
+  return 0;
+}
+
 int32_t gf_ctx_destroy(gf_ctx* ctx) {
   if (!ctx) return 0;
   ctx->c.release();

@@ -132,8 +132,11 @@ __global__ __launch_bounds__(kBlock) void k_mgs_axpy_dot(long n, const double* _
 constexpr int kSweepBlock = 256;
 constexpr long kSweepSpinCap = 1L << 20;
 
-template <int PER>
-__global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
+// The variant without the prefetch asks for three waves per SIMD (at most
+// 168 registers a lane): 768 co-resident blocks, so the 755-block grid of a
+// 1M-element solve fits a process with one hardware queue.
+template <int PER, bool PF>
+__global__ __launch_bounds__(kSweepBlock, PF ? 1 : 3) void k_mgs_sweep(long n, int j, int m,
                                                           const double* __restrict__ V,
                                                           double* __restrict__ w, double* H,
                                                           double* ring, int* flags, double* err,
@@ -145,7 +148,10 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
   const int G = gridDim.x, b = blockIdx.x;
   const long stride = (long)G * kSweepBlock;
   const long base = (long)b * kSweepBlock + threadIdx.x;
-  double wr[PER], vr[PER], vn[PER];
+  // PF: V_{s+1} is prefetched while round s's exchange is in flight (a
+  // third register array); without it the same loads follow the exchange,
+  // with the same values and order (identical bits) and fewer registers
+  double wr[PER], vr[PER], vn[PF ? PER : 1];
   bool broken = false;  // wave 0 only
   // w and V_0 are requested before the ring position, so the two load
   // latencies overlap at the barrier below
@@ -178,12 +184,12 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
       store_ack_wait();  // the previous round's slot emptying has landed
       __hip_atomic_store(&cur[b], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (s < j) {  // prefetch V_{s+1} while the exchange is in flight
+    if (PF && s < j) {  // prefetch V_{s+1} while the exchange is in flight
       const double* vs = V + (long)(s + 1) * n;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const long i = base + k * stride;
-        vn[k] = i < n ? vs[i] : 0.0;
+        vn[PF ? k : 0] = i < n ? vs[i] : 0.0;
       }
     }
     if (threadIdx.x < 64) {
@@ -221,8 +227,12 @@ __global__ __launch_bounds__(kSweepBlock) void k_mgs_sweep(long n, int j, int m,
       for (int k = 0; k < PER; ++k) wr[k] -= h * vr[k];
       if (b == 0 && threadIdx.x == 0) H[(long)s * m + j] = h;
       if (s < j) {
+        const double* vs = V + (long)(s + 1) * n;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) vr[k] = vn[k];
+        for (int k = 0; k < PER; ++k) {
+          const long i = base + k * stride;
+          vr[k] = PF ? vn[PF ? k : 0] : (i < n ? vs[i] : 0.0);
+        }
       }
     } else {
       const double nrm = sqrt(h);
@@ -272,14 +282,14 @@ namespace {
 // hardware queue, so a grid may use at most 1/hw_queues() of the device's
 // co-resident capacity; larger orthogonalisations use one launch per
 // projection.
-template <int PER>
+template <int PER, bool PF>
 int sweep_fits(int G) {
   static std::atomic<int> max_blocks{-1};  // co-resident blocks of this instantiation
   if (max_blocks.load() < 0) {
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mgs_sweep<PER>, kSweepBlock, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mgs_sweep<PER, PF>, kSweepBlock,
+                                                     0) != hipSuccess)
       return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       return 0;
@@ -288,12 +298,11 @@ int sweep_fits(int G) {
   return (long)hw_queues() * G <= max_blocks.load();
 }
 
-template <int PER>
-int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H,
-                 double* err, const int* gate, int cooperative, const GivensArgs* gvp, int G,
-                 hipStream_t st) {
+template <int PER, bool PF>
+int launch_sweep_v(Work& wk, long n, const double* V, int j, int m, double* wv, double* H,
+                   double* err, const int* gate, int cooperative, const GivensArgs* gvp, int G,
+                   hipStream_t st) {
   GivensArgs gv = gvp ? *gvp : GivensArgs{0.0, nullptr, nullptr, nullptr, nullptr, nullptr};
-  if (!sweep_fits<PER>(G)) return 0;
   if (wk.sweep_G != G) {
     // the sweep needs all ring positions of its grid equal and the ring
     // empty at entry; state left by a different grid (or an unknown history)
@@ -309,19 +318,37 @@ int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, do
   if (cooperative) {
     void* args[] = {&n,   &j,   &m,       (void*)&V,  &wv, &H, &wk.ring, &wk.flags, &err,
                     (void*)&gate, &gv};
-    GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER>), dim3(G),
-                                      dim3(kSweepBlock), args, 0, st));
+    GF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_mgs_sweep<PER, PF>),
+                                      dim3(G), dim3(kSweepBlock), args, 0, st));
   } else {
     // G is far below the device's co-resident capacity (checked above), so
     // every block is resident once the kernels ahead of it drain; the waits
     // are bounded regardless
-    hipLaunchKernelGGL((k_mgs_sweep<PER>), dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V, wv, H,
-                       wk.ring, wk.flags, err, gate, gv);
+    hipLaunchKernelGGL((k_mgs_sweep<PER, PF>), dim3(G), dim3(kSweepBlock), 0, st, n, j, m, V,
+                       wv, H, wk.ring, wk.flags, err, gate, gv);
     GF_CHECK_LAUNCH();
   }
   return 1;
 }
+
+// The prefetching sweep where its grid fits the co-residency rule, else the
+// one without the prefetch (fewer registers, more co-resident blocks: at 1M
+// elements its 755 blocks fit a process with one hardware queue), else none
+// (the caller takes one launch per projection).  Both give the same bits.
+std::atomic<int> g_sweep_form{1};
+template <int PER>
+int launch_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H,
+                 double* err, const int* gate, int cooperative, const GivensArgs* gvp, int G,
+                 hipStream_t st) {
+  if (g_sweep_form.load() == 1 && sweep_fits<PER, true>(G))
+    return launch_sweep_v<PER, true>(wk, n, V, j, m, wv, H, err, gate, cooperative, gvp, G, st);
+  if (sweep_fits<PER, false>(G))
+    return launch_sweep_v<PER, false>(wk, n, V, j, m, wv, H, err, gate, cooperative, gvp, G, st);
+  return 0;
+}
 }  // namespace
+
+void set_sweep_form(int form) { g_sweep_form = form == 2 ? 2 : 1; }
 
 int mgs_sweep(Work& wk, long n, const double* V, int j, int m, double* wv, double* H, double* err,
               const int* gate, int cooperative, const GivensArgs* gv, hipStream_t st) {

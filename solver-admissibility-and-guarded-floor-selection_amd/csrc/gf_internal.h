@@ -276,6 +276,9 @@ int ls_reset(Work& w, hipStream_t stream);
 // Line-search form: 1 = one launch where it fits (default), 0 = three launches.
 void set_ls_form(int form);
 int get_ls_form();
+// Gram-Schmidt sweep variant: 1 = prefetching where it fits, else the lean
+// one; 2 = the lean one wherever it fits (same bits)
+void set_sweep_form(int form);
 int axpby_dev(long n, const double* a, double asign, const double* x, const double* b, double* y,
               const int* gate, hipStream_t st);
 int scale_inv(long n, const double* x, const double* h, double thresh, double* y,

@@ -100,6 +100,7 @@ SIGNATURES = {
     "gf_ctx_set_shape": [_P, _I],
     "gf_set_coarse_form": [_I],
     "gf_set_ls_form": [_I],
+    "gf_set_sweep_form": [_I],
     "gf_matvec_f64": [_P, _P, _P, _P, _P],
     "gf_matvec_f32": [_P, _P, _P, _P, _P],
     "gf_residual_f64": [_P, _P, _P, _P, _P, _P],

@@ -600,6 +600,43 @@ def test_coarse_stencil_forms_bitwise_identical():
             assert np.array_equal(a, b)
 
 
+def test_sweep_without_prefetch_bitwise():
+    """The Gram-Schmidt sweep without its prefetch buffer (the variant that
+    lets a 1M-element grid be resident with one hardware queue) gives the
+    prefetching sweep's bits: preconditioner applications (whose inner and
+    root-local FGMRES run sweeps inside the graph) and an outer FGMRES solve
+    with its history at 80x40x20."""
+    import torch
+    from gpu_fem import _lib, policy as gpol
+    from gpu_fem.presets import cantilever
+
+    dims = (80, 40, 20)
+    mask = np.random.default_rng(7).random(int(np.prod(dims))) < 0.35
+    E = _t(np.where(mask, 1.0, 1e-3))
+    outs = []
+    try:
+        for form in (1, 2):
+            _lib.call("gf_set_sweep_form", form)
+            comps = gpol._build_components(cantilever(*dims), **STACK)
+            F, gg = comps[5], comps[7]
+            gg.setup(E)
+            r = _t(np.random.default_rng(9).standard_normal(len(comps[2])))
+            res = [_n(gg.apply(r))]
+            h: list = []
+            x, it, _ = gg.solve(F, tol=1e-6, maxiter=60, restart=300, history=h)
+            res.extend([_n(x), it, h])
+            del comps, gg
+            outs.append(res)
+    finally:
+        _lib.call("gf_set_sweep_form", 1)
+    torch.cuda.synchronize()
+    for a, b in zip(outs[0], outs[1]):
+        if isinstance(a, np.ndarray):
+            assert np.array_equal(a, b)
+        else:
+            assert a == b
+
+
 @pytest.mark.parametrize("dims", [(16, 8, 8), (80, 40, 20)])
 def test_line_search_one_launch_bitwise(dims):
     """The residual line search of every coarse correction in one persistent
