@@ -96,9 +96,10 @@ typedef struct gf_gmg_params {
    * arithmetic order in every mode up to the partial-sum grouping. */
   int32_t mgs_sweep;
   /* iterative coarsest solve (more free DOFs than dense_chol_max): 0 = the
-   * fixed-count Jacobi PCG in two launches per iteration (default), 1 = in
-   * one persistent launch with bounded grid-wide exchanges through uncached
-   * memory (13.3 vs 15.1 us per iteration at 1M elements).
+   * fixed-count Jacobi PCG in two launches per iteration, 1 = in one
+   * persistent launch with bounded grid-wide exchanges through uncached
+   * memory where its grid fits the co-residency rule, else in two launches
+   * (default; 13.3 vs 15.1 us per iteration at 1M elements).
    * The same recurrence; the dot products differ only in partial-sum
    * grouping. */
   int32_t coarse_pcg_persistent;

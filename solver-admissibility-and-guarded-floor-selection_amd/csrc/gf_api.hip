@@ -371,7 +371,7 @@ void gf_gmg_default_params(gf_gmg_params* p) {
   p->coarsest_solve = 0;
   p->graph_apply = 1;
   p->mgs_sweep = 1;
-  p->coarse_pcg_persistent = 0;
+  p->coarse_pcg_persistent = 1;
 }
 
 int32_t gf_gmg_create(gf_gmg** out, gf_ctx* ctx, const gf_gmg_params* params) {

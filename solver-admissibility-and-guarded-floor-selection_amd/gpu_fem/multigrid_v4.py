@@ -193,7 +193,7 @@ class GalerkinMatFreeGMG:
                  root_local_correction_inner_restart=None, inner_krylov_steps: int = 0,
                  inner_krylov_restart=None, coarsest_solve: str = "inverse",
                  graph_apply: bool = True, mgs_sweep=True,
-                 coarse_pcg_persistent: bool = False) -> None:
+                 coarse_pcg_persistent: bool = True) -> None:
         # argument validation (multigrid_v4.py:600-699), same messages
         if smoother_type not in {"chebyshev", "jacobi"}:
             raise ValueError(f"smoother_type must be 'chebyshev' or 'jacobi'; got {smoother_type!r}")

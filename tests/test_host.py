@@ -41,7 +41,7 @@ def test_params_struct_layout_matches_header():
     assert p.n_levels == 4 and p.coarse_smooth_iters == 2 and p.dense_chol_max == 5000
     assert abs(p.cheb_lower_frac - 1 / 30) < 1e-15 and p.coarse_pcg_iters == 80
     assert list(p.level_prec) == [-1] * 8 and p.root_enrichment_weight_floor == 1e-6
-    assert p.mgs_sweep == 1 and p.coarse_pcg_persistent == 0 and p.graph_apply == 1
+    assert p.mgs_sweep == 1 and p.coarse_pcg_persistent == 1 and p.graph_apply == 1
 
 
 def test_library_reports_argument_errors_without_gpu():
