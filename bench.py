@@ -609,7 +609,10 @@ def precision_form(args):
     solution error against the FP64 result (the FP64 form is the one held to
     the oracle), plus the fine-matvec rate at this grid.  A third run smooths
     the fine level with the bf16 matrix-core operator (levels 1.. FP32) and is
-    reported beside them."""
+    reported beside them.  The forms run one at a time on one solver
+    instance, on one hardware queue unless --hw-queues says otherwise (the
+    co-residency rule then admits the 1M Gram-Schmidt sweep as one
+    persistent launch per Arnoldi step)."""
     import numpy as np
     import torch
 
