@@ -56,6 +56,13 @@ def test_library_reports_argument_errors_without_gpu():
     rc = lib.gf_ctx_create(C.byref(h), 2, 2, 2, ke.ctypes.data, free.ctypes.data, 10, 12345)
     assert rc == -2
     assert b"ndof" in lib.gf_last_error()
+    # process-wide kernel-form switches reject unknown forms (no device work)
+    assert lib.gf_set_sweep_form(3) == -2
+    assert b"gf_set_sweep_form" in lib.gf_last_error()
+    assert lib.gf_set_ls_form(2) == -2
+    assert b"gf_set_ls_form" in lib.gf_last_error()
+    assert lib.gf_set_coarse_form(7) == -2
+    assert b"gf_set_coarse_form" in lib.gf_last_error()
 
 
 def test_host_setup_equals_oracle():
