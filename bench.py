@@ -892,11 +892,12 @@ def learned_eval(args, model, pk):
                                                   densities=dens, progress=progress, **pk))
     v_seq, v_plain = acc(plain_seq) / w_seq, acc(plain) / w_plain
     out = {}
-    for name, gate in (("fp32_ungated", 0.0), ("fp32_gated", "auto")):
+    for name, gate, prec in (("fp32_ungated", 0.0, "fp32"), ("fp32_gated", "auto", "fp32"),
+                             ("fp64_gated", "auto", "fp64")):
         progress(f"learned: {name}")
         got, wall = L.run_learned(model, ev, workers=workers, load=args.load, densities=dens,
                                   progress=progress, confidence_threshold=gate, **pk,
-                                  **precision_stack("fp32"))
+                                  **precision_stack(prec))
         c = L.compare(plain, got)
         c.update(value=acc(got) / wall, wall_s=wall,
                  confidence_gate=model.gate_for(dims) if gate == "auto" else 0.0,
