@@ -314,3 +314,24 @@ def test_shipped_network_file():
     assert np.all((z["gate_tau"] > 0) & (z["gate_tau"] <= 1))
     labelled = {tuple(s["dims"]) for f in LABEL_FILES for s in read_labels(f)[0]}
     assert labelled == grids
+
+
+def test_bench_refuses_more_than_one_gpu_and_bad_queue_counts():
+    """bench.py runs on one GPU: --gpus above 1 prints one JSON line that
+    says so and exits 2 before touching a device; a hardware-queue count
+    outside [1, 32] is refused the same way."""
+    import json
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    bench = os.path.join(root, "bench.py")
+    r = subprocess.run([sys.executable, bench, "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in line["error"]
+    r = subprocess.run([sys.executable, bench, "--hw-queues", "40"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "--hw-queues must be in [1, 32]" in r.stderr
